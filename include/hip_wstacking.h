@@ -1,0 +1,223 @@
+/*
+ * hip_wstacking.h - C ABI of the MI355X (gfx950) w-stacking gridder/degridder.
+ *
+ * The entry points mirror the call protocol of the reference core
+ * WStackingGridder<T> (gridding/wstackinggridder.h:44-84):
+ *
+ *   invert:  hws_create -> hws_prepare_w_layers -> per pass
+ *            { hws_start_inversion_pass; hws_add_data*; hws_finish_inversion_pass }
+ *            -> hws_finalize_image
+ *   predict: hws_create -> hws_prepare_w_layers -> hws_initialize_prediction
+ *            -> per pass { hws_start_prediction_pass; hws_sample_data* }
+ *
+ * plus one-call device-resident forms (hws_invert_device / hws_predict_device)
+ * that run every pass, all channels and all images of a batch in one sequence
+ * of launches, with the uv-tile binning of the visibilities kept in a plan
+ * that invert and predict share.
+ *
+ * Conventions (identical to the reference):
+ *   - images are row-major, x fastest; pixel (x, y) has
+ *     l = (W/2 - x) * dl + l_shift and m = (y - H/2) * dm + m_shift;
+ *   - uvw are in metres, one triple per row; wavelengths in metres per
+ *     channel (the value BandData::ChannelWavelength(ch) returns);
+ *   - visibilities are complex64 (interleaved float re, im) laid out
+ *     [row][channel][image];
+ *   - a context grids n_images images that share uvw (e.g. the correlations
+ *     XX, XY, YX, YY); bit i of complex_mask marks image i as a complex image
+ *     (the reference's SetIsComplex, used for XY/YX,
+ *     scheduling/griddingtaskmanager.cpp:106-107);
+ *   - predicted samples that no pass covers, or that fall off the grid, are
+ *     NaN (gridding/wstackinggridder.cpp:626-627, 677-678, 685-687).
+ *
+ * Every entry point returns an hws_status; on failure hws_last_error() gives a
+ * message. Nothing throws across this boundary. A context is used by one
+ * thread at a time; several contexts may run concurrently on one device.
+ */
+#ifndef HIP_WSTACKING_H
+#define HIP_WSTACKING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hws_status {
+  HWS_OK = 0,
+  HWS_ERR_INVALID_ARGUMENT = 1,
+  HWS_ERR_HIP = 2,
+  HWS_ERR_OUT_OF_MEMORY = 3,
+  HWS_ERR_UNSUPPORTED = 4,
+  HWS_ERR_BAD_STATE = 5
+} hws_status;
+
+/* Values of GriddingKernelMode (gridding/gridmode.h:6-44). */
+typedef enum hws_kernel_mode {
+  HWS_KERNEL_NEAREST_NEIGHBOUR = 0,
+  HWS_KERNEL_KAISER_BESSEL = 1,
+  HWS_KERNEL_KAISER_BESSEL_NO_SINC = 2,
+  HWS_KERNEL_RECTANGULAR = 3,
+  HWS_KERNEL_GAUSSIAN = 4,
+  HWS_KERNEL_GAUSSIAN_NO_SINC = 5,
+  HWS_KERNEL_BLACKMAN_NUTTALL = 6,
+  HWS_KERNEL_BLACKMAN_NUTTALL_NO_SINC = 7,
+  HWS_KERNEL_BLACKMAN_HARRIS = 8 /* rejected, as in the reference */
+} hws_kernel_mode;
+
+typedef enum hws_precision {
+  /* One launch per channel and image; uvw scaling, sample position, layer
+   * index and image-domain phase in FP32; FP32 kernel tables; every tap is a
+   * global FP32 atomic. The plain GPU form. */
+  HWS_PRECISION_PLAIN_FP32 = 0,
+  /* Batched uv-tile binned gridding with LDS subgrids; FP64 uvw scaling,
+   * position, layer index and phase arguments (as the reference computes
+   * them); FP32 kernel tables; FP64 sin/cos for the w-correction; Kahan
+   * compensated FP32 layer->image sums. */
+  HWS_PRECISION_REFERENCE = 1,
+  /* As REFERENCE, but FP16 kernel tables (hi + lo pair), FP32 accumulation,
+   * and w-correction phases reduced modulo one turn in FP64 before an FP32
+   * sincospi. */
+  HWS_PRECISION_MIXED = 2
+} hws_precision;
+
+typedef struct hws_config {
+  size_t width, height;         /* even, 7-smooth, <= 16384 */
+  double pixel_size_x, pixel_size_y; /* radians */
+  size_t kernel_size;           /* odd, <= 15 */
+  size_t oversampling;          /* <= 1023 */
+  int kernel_mode;              /* hws_kernel_mode */
+  int precision;                /* hws_precision */
+  int device;                   /* HIP device ordinal */
+  size_t n_images;              /* 1..4 */
+  uint32_t complex_mask;        /* bit i: image i is complex */
+  double l_shift, m_shift;      /* SetDenormalPhaseCentre */
+  /* 1: multiply visibilities by exp(-2 pi i (u l_shift + v m_shift)/lambda)
+   * before gridding and by the conjugate after degridding, as WSMSGridder
+   * does (gridding/wsmsgridder.cpp:185-190, 328-332). 0: only the n' shift
+   * of WStackingGridder::SetDenormalPhaseCentre. */
+  int rotate_visibilities;
+  size_t tile_size;             /* uv tile edge for binning; 0 = default */
+} hws_config;
+
+typedef struct hws_gridder hws_gridder;
+
+/* Counters of the last invert/predict on a context. */
+typedef struct hws_stats {
+  uint64_t samples_in;        /* (row, channel) samples handed in */
+  uint64_t samples_gridded;   /* accepted on a layer of a pass and on the grid */
+  uint64_t samples_off_grid;  /* rejected by -W/2 < x <= W/2, -H/2 < y <= H/2 */
+  uint64_t samples_off_layer; /* w outside [minW - dw/2, maxW + dw/2] */
+  uint64_t bins_used;         /* non-empty (layer, tile) bins */
+  uint64_t chunks;            /* work items of the binned grid/degrid kernels */
+  double ms_bin, ms_grid, ms_fft, ms_project, ms_total; /* device times */
+} hws_stats;
+
+/* Replaces: WStackingGridder<T>::WStackingGridder (gridding/wstackinggridder.h:120-122,
+ * cpp:16-41) together with SetGridMode/SetIsComplex/SetDenormalPhaseCentre
+ * (h:470-508). */
+hws_status hws_create(const hws_config* config, hws_gridder** out);
+/* Replaces: ~WStackingGridder (cpp:53-61). */
+void hws_destroy(hws_gridder* g);
+/* Message for the last failed call on g (or a global message if g is NULL). */
+const char* hws_last_error(const hws_gridder* g);
+const char* hws_status_string(hws_status s);
+
+/* Replaces: WStackingGridder::PrepareWLayers (h:158, cpp:63-116). max_mem is a
+ * device-memory budget in bytes for the layers of one pass (0 = 75% of the
+ * free device memory). Returns the pass count in *n_passes. Layer ranges per
+ * pass follow layerRangeStart (h:566-568). */
+hws_status hws_prepare_w_layers(hws_gridder* g, size_t n_w_layers,
+                                double max_mem, double min_w, double max_w,
+                                size_t* n_passes);
+/* Replaces: WToLayer (h:182-193) for image kind is_complex. Returns
+ * SIZE_MAX where the reference casts a negative value to size_t. */
+size_t hws_w_to_layer(const hws_gridder* g, double w_lambda, int is_complex);
+/* Replaces: LayerToW (h:202-212). */
+double hws_layer_to_w(const hws_gridder* g, size_t layer, int is_complex);
+/* Replaces: layerRangeStart (h:566-568): layers [*start, *end) of a pass. */
+hws_status hws_pass_layer_range(const hws_gridder* g, size_t pass,
+                                size_t* start, size_t* end);
+
+/* Replaces: StartInversionPass (h:311, cpp:127-137). */
+hws_status hws_start_inversion_pass(hws_gridder* g, size_t pass);
+/* Replaces: AddData (h:288-289, cpp:971-982) for a block of rows, i.e.
+ * AddDataSample (cpp:517-597) for every (row, channel) with
+ * u = u_m / wavelength etc. Host pointers; vis must be pre-weighted. */
+hws_status hws_add_data(hws_gridder* g, size_t nrow, size_t nchan,
+                        const double* uvw_m, const double* wavelengths,
+                        const float* vis);
+/* Replaces: AddDataSample (h:300-301) for n samples whose uvw are already in
+ * wavelengths: uvw_lambda [n][3], vis [n][n_images]. */
+hws_status hws_add_samples(hws_gridder* g, size_t n, const double* uvw_lambda,
+                           const float* vis);
+/* Replaces: FinishInversionPass (h:319, cpp:323-336). */
+hws_status hws_finish_inversion_pass(hws_gridder* g);
+/* Replaces: FinalizeImage (h:326, cpp:691-715) + RealImageFloat /
+ * ImaginaryImageFloat (h:412-435, cpp:996-1022). real: [n_images][H][W];
+ * imag: [n_images][H][W] or NULL (filled for complex images only). */
+hws_status hws_finalize_image(hws_gridder* g, double multiplication_factor,
+                              float* real, float* imag);
+
+/* Replaces: InitializePrediction (h:341-357, cpp:829-849). real/imag as in
+ * hws_finalize_image; imag is required iff some image is complex. */
+hws_status hws_initialize_prediction(hws_gridder* g, const float* real,
+                                     const float* imag);
+/* Replaces: StartPredictionPass (h:366, cpp:139-157). */
+hws_status hws_start_prediction_pass(hws_gridder* g, size_t pass);
+/* Replaces: SampleData (h:385-386, cpp:984-994) for a block of rows; vis_out
+ * [nrow][nchan][n_images]; NaN where this pass predicts nothing. */
+hws_status hws_sample_data(hws_gridder* g, size_t nrow, size_t nchan,
+                           const double* uvw_m, const double* wavelengths,
+                           float* vis_out);
+/* Replaces: SampleDataSample (h:400-401) for n samples in wavelengths. */
+hws_status hws_sample_samples(hws_gridder* g, size_t n,
+                              const double* uvw_lambda, float* vis_out);
+
+/* Replaces: GetGriddedUVLayer (h:518-520): copies layer `layer` (relative to
+ * the current pass) of image `image` as [H][W] complex64 to host memory. */
+hws_status hws_get_gridded_uv_layer(hws_gridder* g, size_t image,
+                                    size_t layer, float* out);
+/* Replaces: the static GetKernel (h:530-531, cpp:387-423). */
+hws_status hws_get_kernel(int kernel_mode, size_t oversampling, size_t size,
+                          double* kernel);
+
+/* ---- device-resident batched forms (no reference counterpart as single
+ * calls; they run the same protocol as above over every pass) ----
+ * All pointers are device pointers. Work is ordered on the stream set with
+ * hws_set_stream (default: a stream the context owns); calls return once the
+ * work is enqueued, except where a host-visible result is needed.
+ * The context must have been prepared with hws_prepare_w_layers. */
+hws_status hws_set_stream(hws_gridder* g, void* hip_stream);
+/* Bin (row, channel) samples by (w-layer, uv tile) for both image kinds; the
+ * plan is kept in the context and used by the *_planned calls below. */
+hws_status hws_plan_device(hws_gridder* g, size_t nrow, size_t nchan,
+                           const double* d_uvw_m, const double* d_wavelengths);
+/* Whole invert over all passes: real/imag [n_images][H][W] device pointers,
+ * imag may be NULL when no image is complex. */
+hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
+                                     double multiplication_factor,
+                                     float* d_real, float* d_imag);
+/* Whole predict over all passes: d_vis_out [nrow][nchan][n_images]. */
+hws_status hws_predict_planned_device(hws_gridder* g, const float* d_real,
+                                      const float* d_imag, float* d_vis_out);
+/* hws_plan_device followed by the planned call. */
+hws_status hws_invert_device(hws_gridder* g, size_t nrow, size_t nchan,
+                             const double* d_uvw_m,
+                             const double* d_wavelengths, const float* d_vis,
+                             double multiplication_factor, float* d_real,
+                             float* d_imag);
+hws_status hws_predict_device(hws_gridder* g, size_t nrow, size_t nchan,
+                              const double* d_uvw_m,
+                              const double* d_wavelengths, const float* d_real,
+                              const float* d_imag, float* d_vis_out);
+/* Counters and device timings of the last invert/predict. */
+hws_status hws_get_stats(const hws_gridder* g, hws_stats* out);
+/* Wait for all work of the context. */
+hws_status hws_synchronize(hws_gridder* g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIP_WSTACKING_H */

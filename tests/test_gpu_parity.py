@@ -1,0 +1,300 @@
+"""Parity of the HIP w-stacking core (C ABI) with the oracle: the CPU
+restatement of the reference WStackingGridder<float|double> on the same
+seeded inputs, and the FP64 direct FT.
+
+Tolerances (relative L2 unless stated), set from the oracle's own
+float-vs-double spread (~1e-6) and the precision of each form:
+  layers:  reference/mixed 2e-6; plain 2e-3 (FP32 sample positions)
+  images:  reference 1e-5, mixed 3e-5, plain 3e-3
+  predict: reference 1e-5, mixed 3e-5, plain 3e-3
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_IMG = {"reference": 1e-5, "mixed": 3e-5, "plain": 3e-3}
+TOL_LAYER = {"reference": 2e-6, "mixed": 2e-6, "plain": 2e-3}
+C_LIGHT = 299792458.0
+
+
+def rel(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+@pytest.fixture(scope="module")
+def hws():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import wsclean_amd
+    return wsclean_amd
+
+
+def case(seed, width, nrow=300, nchan=3, wmax_m=60.0, frac=0.42, dl=0.004, nimg=1):
+    """Random rows (metres) and channels whose uv cover `frac` of the grid."""
+    rng = np.random.default_rng(seed)
+    freqs = np.linspace(120e6, 150e6, nchan)
+    wl = C_LIGHT / freqs
+    umax = frac / dl * wl.min()
+    uvw = np.column_stack([rng.uniform(-umax, umax, nrow), rng.uniform(-umax, umax, nrow),
+                           rng.uniform(-wmax_m, wmax_m, nrow)])
+    vis = (rng.standard_normal((nrow, nchan, nimg)) +
+           1j * rng.standard_normal((nrow, nchan, nimg))).astype(np.complex64)
+    return uvw, wl, vis, dl
+
+
+def lambda_samples(uvw, wl):
+    return (uvw[:, None, :] / wl[None, :, None]).reshape(-1, 3)
+
+
+def oracle_invert(oracle, w, h, dl, dm, uvw_l, vis, nw, minw, maxw, cplx=False, mode="kb",
+                  prec="float", shift=(0.0, 0.0)):
+    g = oracle.RefGridder(w, h, dl, dm, 2, 7, 63, prec)
+    if mode != "kb":
+        g.set_grid_mode(mode)
+    g.set_is_complex(cplx)
+    if shift != (0.0, 0.0):
+        g.set_denormal_phase_centre(*shift)
+    g.prepare_w_layers(nw, 1e12, minw, maxw)
+    g.start_inversion_pass(0)
+    g.add_samples(uvw_l, vis)
+    layers = [g.layer(i) for i in range(nw)]
+    g.finish_inversion_pass()
+    g.finalize_image(1.0)
+    return g.image(imag=cplx), layers
+
+
+def oracle_predict(oracle, w, h, dl, dm, uvw_l, real, imag, nw, minw, maxw, cplx=False,
+                   mode="kb", prec="float", shift=(0.0, 0.0)):
+    g = oracle.RefGridder(w, h, dl, dm, 2, 7, 63, prec)
+    if mode != "kb":
+        g.set_grid_mode(mode)
+    g.set_is_complex(cplx)
+    if shift != (0.0, 0.0):
+        g.set_denormal_phase_centre(*shift)
+    g.prepare_w_layers(nw, 1e12, minw, maxw)
+    g.initialize_prediction(real, imag)
+    g.start_prediction_pass(0)
+    return g.sample(uvw_l)
+
+
+def gpu_invert(hws, w, h, dl, dm, uvw, wl, vis, nw, minw, maxw, precision, complex_images=(),
+               mode="kb", layers=False, max_mem=0.0, **kw):
+    nimg = vis.shape[2]
+    g = hws.WStackingGridder(w, h, dl, dm, 7, 63, kernel_mode=mode, precision=precision,
+                             n_images=nimg, complex_images=complex_images, **kw)
+    npass = g.prepare_w_layers(nw, max_mem, minw, maxw)
+    lay = None
+    for p in range(npass):
+        g.start_inversion_pass(p)
+        g.add_data(uvw, wl, vis)
+        if layers and p == 0:
+            lay = [[g.get_gridded_uv_layer(i, image=k) for i in range(nw)] for k in range(nimg)]
+        g.finish_inversion_pass()
+    re, im = g.finalize_image(1.0)
+    return re, im, lay, npass
+
+
+@pytest.mark.parametrize("precision", ["reference", "mixed", "plain"])
+def test_invert_layers_and_image_match_oracle(hws, oracle, precision):
+    w = 64
+    uvw, wl, vis, dl = case(1, w)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    nw = 8
+    (img_o, lay_o) = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis.reshape(-1), nw, 0.0, maxw)
+    re, _, lay, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, nw, 0.0, maxw, precision,
+                               layers=True)
+    for i in range(nw):
+        if np.abs(lay_o[i]).sum() == 0:
+            assert np.abs(lay[0][i]).sum() == 0
+            continue
+        assert rel(lay[0][i], lay_o[i]) < TOL_LAYER[precision], (precision, i)
+    e = rel(re[0], img_o)
+    print(f"invert {precision}: image rel L2 vs oracle float = {e:.2e}")
+    assert e < TOL_IMG[precision]
+
+
+@pytest.mark.parametrize("precision", ["reference", "mixed", "plain"])
+def test_predict_matches_oracle(hws, oracle, precision):
+    w = 64
+    uvw, wl, _, dl = case(2, w)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    rng = np.random.default_rng(22)
+    model = rng.standard_normal((w, w)).astype(np.float32)
+    v_o = oracle_predict(oracle, w, w, dl, dl, uvw_l, model, None, 8, 0.0, maxw)
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision=precision)
+    g.prepare_w_layers(8, 0.0, 0.0, maxw)
+    g.initialize_prediction(model)
+    g.start_prediction_pass(0)
+    v = g.sample_data(uvw, wl).reshape(-1)
+    assert np.array_equal(np.isnan(v), np.isnan(v_o))
+    ok = ~np.isnan(v_o)
+    e = rel(v[ok], v_o[ok])
+    print(f"predict {precision}: rel L2 vs oracle float = {e:.2e}")
+    assert e < TOL_IMG[precision]
+
+
+def test_complex_and_real_images_together(hws, oracle):
+    # XX (real) and XY (complex) sharing uvw, gridded in one context
+    w = 64
+    uvw, wl, vis, dl = case(3, w, nimg=2)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    re, im, _, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 8, 0.0, maxw, "reference",
+                              complex_images=(1,))
+    img0, _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis[:, :, 0].reshape(-1), 8, 0.0, maxw)
+    (img1r, img1i), _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis[:, :, 1].reshape(-1), 8,
+                                      0.0, maxw, cplx=True)
+    assert rel(re[0], img0) < TOL_IMG["reference"]
+    assert rel(re[1], img1r) < TOL_IMG["reference"]
+    assert rel(im[1], img1i) < TOL_IMG["reference"]
+    # prediction of both
+    rng = np.random.default_rng(33)
+    mre = rng.standard_normal((2, w, w)).astype(np.float32)
+    mim = rng.standard_normal((2, w, w)).astype(np.float32)
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision="reference", n_images=2,
+                             complex_images=(1,))
+    g.prepare_w_layers(8, 0.0, 0.0, maxw)
+    g.initialize_prediction(mre, mim)
+    g.start_prediction_pass(0)
+    v = g.sample_data(uvw, wl)
+    v0 = oracle_predict(oracle, w, w, dl, dl, uvw_l, mre[0], None, 8, 0.0, maxw)
+    v1 = oracle_predict(oracle, w, w, dl, dl, uvw_l, mre[1], mim[1], 8, 0.0, maxw, cplx=True)
+    assert rel(v[:, :, 0].reshape(-1), v0) < TOL_IMG["reference"]
+    assert rel(v[:, :, 1].reshape(-1), v1) < TOL_IMG["reference"]
+
+
+def test_nearest_neighbour_mode(hws, oracle):
+    w = 64
+    uvw, wl, vis, dl = case(4, w)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    img_o, lay_o = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis.reshape(-1), 4, 0.0, maxw,
+                                 mode="nn")
+    re, _, lay, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 4, 0.0, maxw, "reference",
+                               mode="nn", layers=True)
+    for i in range(4):
+        np.testing.assert_allclose(lay[0][i], lay_o[i], rtol=1e-6, atol=1e-5)
+    assert rel(re[0], img_o) < TOL_IMG["reference"]
+
+
+def test_shifted_phase_centre_with_rotation(hws, oracle):
+    # WSMSGridder rotates samples by exp(-2 pi i (u dl + v dm)/lambda) before
+    # gridding (gridding/wsmsgridder.cpp:185-190) and n' uses the shifted l, m
+    w = 64
+    uvw, wl, vis, dl = case(5, w)
+    shift = (0.013, -0.008)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    ph = -2 * np.pi * (uvw_l[:, 0] * shift[0] + uvw_l[:, 1] * shift[1])
+    rot = (vis.reshape(-1).astype(np.complex128) * np.exp(1j * ph)).astype(np.complex64)
+    img_o, _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, rot, 8, 0.0, maxw, shift=shift)
+    re, _, _, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 8, 0.0, maxw, "reference",
+                             l_shift=shift[0], m_shift=shift[1], rotate_visibilities=True)
+    assert rel(re[0], img_o) < TOL_IMG["reference"]
+
+
+def test_multipass_equals_single_pass(hws):
+    w = 64
+    uvw, wl, vis, dl = case(6, w)
+    maxw = np.abs(lambda_samples(uvw, wl)[:, 2]).max()
+    one, _, _, n1 = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 12, 0.0, maxw, "reference")
+    many, _, _, n2 = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 12, 0.0, maxw, "reference",
+                                max_mem=w * w * 8 * 5)
+    assert n1 == 1 and n2 == 3
+    assert rel(many, one) < 1e-6
+
+
+def test_nan_contract_off_grid_and_other_pass(hws):
+    w = 32
+    dl = 0.01
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision="reference")
+    npass = g.prepare_w_layers(8, w * w * 8 * 4, 1.0, 100.0)
+    assert npass == 2
+    g.initialize_prediction(np.ones((w, w), np.float32))
+    uvw = np.array([[0.0, 0.0, 1.0], [0.0, 0.0, 99.0], [1e6, 0.0, 1.0]])
+    g.start_prediction_pass(0)
+    v0 = g.sample_samples(uvw)[:, 0]
+    assert np.isfinite(v0[0]) and np.isnan(v0[1]) and np.isnan(v0[2])
+    g.start_prediction_pass(1)
+    v1 = g.sample_samples(uvw)[:, 0]
+    assert np.isnan(v1[0]) and np.isfinite(v1[1]) and np.isnan(v1[2])
+
+
+@pytest.mark.parametrize("shape", [(96, 96), (80, 96)])
+def test_non_power_of_two_sizes(hws, oracle, shape):
+    w, h = shape
+    uvw, wl, vis, dl = case(7, min(w, h))
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    img_o, _ = oracle_invert(oracle, w, h, dl, dl, uvw_l, vis.reshape(-1), 6, 0.0, maxw)
+    re, _, _, _ = gpu_invert(hws, w, h, dl, dl, uvw, wl, vis, 6, 0.0, maxw, "reference")
+    assert rel(re[0], img_o) < TOL_IMG["reference"]
+
+
+def test_device_forms_match_host_protocol(hws):
+    import torch
+    w = 64
+    uvw, wl, vis, dl = case(8, w, nimg=4)
+    maxw = np.abs(lambda_samples(uvw, wl)[:, 2]).max()
+    cimg = (1, 2)
+    re_h, im_h, _, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 8, 0.0, maxw, "mixed",
+                                  complex_images=cimg)
+    dev = torch.device("cuda:0")
+    g = hws.DeviceGridder(w, w, dl, dl, 7, 63, precision="mixed", n_images=4,
+                          complex_images=cimg)
+    g.prepare_w_layers(8, 0.0, maxw)
+    d_uvw = torch.from_numpy(uvw).to(dev)
+    d_wl = torch.from_numpy(wl).to(dev)
+    d_vis = torch.from_numpy(vis).to(dev)
+    re = torch.zeros((4, w, w), dtype=torch.float32, device=dev)
+    im = torch.zeros_like(re)
+    g.plan(d_uvw, d_wl)
+    g.invert(d_vis, 1.0, re, im)
+    g.synchronize()
+    assert rel(re.cpu().numpy(), re_h) < 1e-6
+    assert rel(im.cpu().numpy()[1:3], im_h[1:3]) < 1e-6
+    out = torch.empty_like(d_vis)
+    g.predict(re, im, out)
+    g.synchronize()
+    h = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision="mixed", n_images=4,
+                             complex_images=cimg)
+    h.prepare_w_layers(8, 0.0, 0.0, maxw)
+    h.initialize_prediction(re.cpu().numpy(), im.cpu().numpy())
+    h.start_prediction_pass(0)
+    v_h = h.sample_data(uvw, wl)
+    v_d = out.cpu().numpy()
+    assert np.array_equal(np.isnan(v_h), np.isnan(v_d))
+    ok = ~np.isnan(v_h)
+    assert rel(v_d[ok], v_h[ok]) < 1e-6
+    st = g.stats()
+    assert st["samples_in"] == uvw.shape[0] * wl.shape[0]
+
+
+@pytest.mark.parametrize("precision", ["reference", "mixed"])
+def test_gpu_predict_is_adjoint_of_invert(hws, precision):
+    w = 64
+    uvw, wl, vis, dl = case(9, w, nimg=2)
+    maxw = np.abs(lambda_samples(uvw, wl)[:, 2]).max()
+    re, im, _, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 8, 0.0, maxw, precision,
+                              complex_images=(1,))
+    rng = np.random.default_rng(99)
+    mre = rng.standard_normal((2, w, w)).astype(np.float32)
+    mim = rng.standard_normal((2, w, w)).astype(np.float32)
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision=precision, n_images=2,
+                             complex_images=(1,))
+    g.prepare_w_layers(8, 0.0, 0.0, maxw)
+    g.initialize_prediction(mre, mim)
+    g.start_prediction_pass(0)
+    a = g.sample_data(uvw, wl)
+    for k in range(2):
+        lhs = np.sum(re[k].astype(np.float64) * mre[k])
+        if k == 1:
+            lhs += np.sum(im[k].astype(np.float64) * mim[k])
+        rhs = np.real(np.vdot(vis[:, :, k].astype(np.complex128).ravel(),
+                              a[:, :, k].astype(np.complex128).ravel()))
+        assert abs(lhs - rhs) < 1e-5 * abs(lhs), (k, lhs, rhs)

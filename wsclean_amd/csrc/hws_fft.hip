@@ -1,0 +1,487 @@
+// LDS-staged mixed-radix Stockham FFT for the per-layer 2-D transforms
+// (the reference runs fftw[f]_plan_dft_2d per layer,
+// gridding/wstackinggridder.cpp:200-239, 282-321).
+//
+// A 2-D transform of a batch of H x W layers is two launches:
+//   rows:    one workgroup per (row block, layer); a row of W complex values
+//            is loaded with coalesced float2 loads, transformed through the
+//            Stockham stages in LDS, and written back in place;
+//   columns: one workgroup per (column block of Bc columns, layer); the Bc
+//            columns are gathered into LDS as [Bc][H+1] (padded to spread
+//            banks), transformed, and scattered back. Column blocks that
+//            share 128-byte lines are mapped onto the same XCD so that they
+//            share its L2.
+// Each Stockham stage reads its R inputs into registers, synchronises, and
+// writes its R outputs in natural (autosorted) order, so one LDS buffer per
+// transform suffices. Radices 8, 4, 2, 3, 5, 7.
+#include "hws_internal.h"
+
+#include <cmath>
+
+namespace hws {
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+// multiply by sign * i
+template <int SIGN>
+__device__ __forceinline__ float2 mulI(float2 a) {
+  return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int SIGN, int R>
+struct Butterfly;
+
+template <int SIGN>
+struct Butterfly<SIGN, 2> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int SIGN>
+struct Butterfly<SIGN, 4> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    float2 b0 = cadd(v[1], v[3]), b1 = mulI<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+  }
+};
+
+template <int SIGN>
+struct Butterfly<SIGN, 8> {
+  __device__ __forceinline__ static void run(float2* v) {
+    // radix-2 over even/odd, then radix-4 on each half
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Butterfly<SIGN, 4>::run(e);
+    Butterfly<SIGN, 4>::run(o);
+    const float h = 0.70710678118654752440f;
+    // o[k] *= w8^k, w8 = exp(SIGN * 2 pi i / 8)
+    o[1] = make_float2(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
+    o[2] = mulI<SIGN>(o[2]);
+    o[3] = make_float2(h * (-o[3].x - SIGN * o[3].y), h * (-o[3].y + SIGN * o[3].x));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// Odd radix: direct DFT with exp(SIGN 2 pi i k/R) constants.
+template <int SIGN, int R>
+struct Butterfly {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 res[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int m = (r * k) % R;
+        float s, c;
+        // constants folded at compile time
+        c = (float)__builtin_cos(2.0 * 3.14159265358979323846 * m / R);
+        s = (float)(SIGN * __builtin_sin(2.0 * 3.14159265358979323846 * m / R));
+        acc = cadd(acc, cmul(v[r], make_float2(c, s)));
+      }
+      res[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = res[k];
+  }
+};
+
+constexpr int kMaxValues = 16;  // complex values held per thread per stage
+
+// One Stockham stage of radix R over nfft transforms of length n stored at
+// buf + f * pitch. Butterfly b of transform f reads x[b + r*B] (B = n/R),
+// applies twiddle w^(jm*r) with jm = b % ns, and writes
+// y[(b/ns)*ns*R + jm + r*ns].
+template <int SIGN, int R>
+__device__ __forceinline__ void stockhamStage(float2* buf, int n, int pitch,
+                                              int nfft, int ns,
+                                              const float2* __restrict__ tw,
+                                              int tid, int nthr) {
+  constexpr int kBatches = kMaxValues / R;
+  const int B = n / R;
+  const int total = B * nfft;
+  const int twStep = n / (ns * R);
+  float2 v[kBatches][R];
+#pragma unroll
+  for (int q = 0; q < kBatches; ++q) {
+    const int t = tid + q * nthr;
+    if (t < total) {
+      const int f = t / B, b = t - f * B;
+      const int jm = b % ns;
+      const float2* x = buf + f * pitch;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 val = x[b + r * B];
+        if (r != 0 && jm != 0) val = cmul(val, tw[jm * r * twStep]);
+        v[q][r] = val;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kBatches; ++q) {
+    const int t = tid + q * nthr;
+    if (t < total) {
+      const int f = t / B, b = t - f * B;
+      Butterfly<SIGN, R>::run(v[q]);
+      const int jm = b % ns;
+      const int idxD = (b / ns) * ns * R + jm;
+      float2* y = buf + f * pitch;
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[idxD + r * ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Stage radices are packed 4 bits per stage (stage 0 lowest) so that the
+// per-stage dispatch reads a scalar rather than an indexed argument array.
+template <int SIGN>
+__device__ __forceinline__ void fftLds(float2* buf, int pitch, int nfft, int n,
+                       unsigned long long radices, const float2* tw, int tid,
+                       int nthr) {
+  int ns = 1;
+  while (radices != 0ull) {
+    const int R = int(radices & 15ull);
+    radices >>= 4;
+    switch (R) {
+      case 8: stockhamStage<SIGN, 8>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
+      case 4: stockhamStage<SIGN, 4>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
+      case 2: stockhamStage<SIGN, 2>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
+      case 3: stockhamStage<SIGN, 3>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
+      case 5: stockhamStage<SIGN, 5>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
+      case 7: stockhamStage<SIGN, 7>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
+      default: break;
+    }
+    ns *= R;
+  }
+}
+
+// ---- power-of-two lengths: every index is a compile-time shift or mask.
+// Stage radices are 8 while the remaining length allows, then 4, then 2.
+template <int SIGN, int R, int N, int NS, int NT, int NFFT>
+__device__ __forceinline__ void pow2Stage(float2* buf, int pitch, int nfft,
+                                          const float2* __restrict__ tw, int tid) {
+  constexpr int B = N / R;
+  constexpr int TOTAL = B * NFFT;
+  constexpr int BATCHES = (TOTAL + NT - 1) / NT;
+  constexpr int TWSTEP = N / (NS * R);
+  float2 v[BATCHES][R];
+#pragma unroll
+  for (int q = 0; q < BATCHES; ++q) {
+    const int t = tid + q * NT;
+    const int f = t / B, b = t % B;
+    if (f < nfft) {
+      const int jm = b % NS;
+      const float2* x = buf + f * pitch;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 val = x[b + r * B];
+        if (NS > 1 && r != 0) val = cmul(val, tw[jm * r * TWSTEP]);
+        v[q][r] = val;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BATCHES; ++q) {
+    const int t = tid + q * NT;
+    const int f = t / B, b = t % B;
+    if (f < nfft) {
+      Butterfly<SIGN, R>::run(v[q]);
+      const int jm = b % NS;
+      float2* y = buf + f * pitch + (b / NS) * NS * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[r * NS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int SIGN, int N, int NT, int NFFT, int NS = 1>
+__device__ __forceinline__ void pow2Fft(float2* buf, int pitch, int nfft,
+                                        const float2* __restrict__ tw, int tid) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = (REM % 8 == 0) ? 8 : (REM % 4 == 0) ? 4 : 2;
+    pow2Stage<SIGN, R, N, NS, NT, NFFT>(buf, pitch, nfft, tw, tid);
+    pow2Fft<SIGN, N, NT, NFFT, NS * R>(buf, pitch, nfft, tw, tid);
+  }
+}
+
+// rows of length N, NFFT rows per block (the last block may hold fewer)
+template <int SIGN, int N, int NT, int NFFT>
+__global__ void __launch_bounds__(NT)
+fftRowsPow2Kernel(float2* __restrict__ data, int H, const float2* __restrict__ tw) {
+  __shared__ float2 lds[NFFT * N];
+  const int tid = threadIdx.x;
+  const int row0 = blockIdx.x * NFFT;
+  const int nrows = min(NFFT, H - row0);
+  float2* base = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(row0) * N;
+  const int total = nrows * N;
+  for (int i = tid; i < total; i += NT) lds[i] = base[i];
+  __syncthreads();
+  pow2Fft<SIGN, N, NT, NFFT>(lds, N, nrows, tw, tid);
+  for (int i = tid; i < total; i += NT) base[i] = lds[i];
+}
+
+// columns of length N (= H), NFFT columns per block, LDS [NFFT][N + 1]
+template <int SIGN, int N, int NT, int NFFT>
+__global__ void __launch_bounds__(NT)
+fftColsPow2Kernel(float2* __restrict__ data, int W, int nColBlocks,
+                  const float2* __restrict__ tw) {
+  __shared__ float2 lds[NFFT * (N + 1)];
+  const int tid = threadIdx.x;
+  int blk = blockIdx.x;
+  if ((nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
+  const int c0 = blk * NFFT;
+  const int ncols = min(NFFT, W - c0);
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + c0;
+  const int total = ncols * N;
+  for (int i = tid; i < total; i += NT) {
+    const int y = i / ncols, c = i - y * ncols;
+    lds[c * (N + 1) + y] = base[size_t(y) * W + c];
+  }
+  __syncthreads();
+  pow2Fft<SIGN, N, NT, NFFT>(lds, N + 1, ncols, tw, tid);
+  for (int i = tid; i < total; i += NT) {
+    const int y = i / ncols, c = i - y * ncols;
+    base[size_t(y) * W + c] = lds[c * (N + 1) + y];
+  }
+}
+
+// rows: grid (ceil(H / rowsPerBlock), nbatch); each block transforms
+// rowsPerBlock consecutive rows of one layer.
+template <int SIGN>
+__global__ void __launch_bounds__(1024)
+fftRowsKernel(float2* __restrict__ data, int W, int H, int rowsPerBlock,
+              unsigned long long radices, const float2* __restrict__ tw) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int row0 = blockIdx.x * rowsPerBlock;
+  const int nrows = min(rowsPerBlock, H - row0);
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) +
+                 size_t(row0) * size_t(W);
+  const int total = nrows * W;
+  for (int i = tid; i < total; i += nthr) lds[i] = base[i];
+  __syncthreads();
+  fftLds<SIGN>(lds, W, nrows, W, radices, tw, tid, nthr);
+  for (int i = tid; i < total; i += nthr) base[i] = lds[i];
+}
+
+// columns: grid (nColBlocks, nbatch); block handles columns
+// [c0, c0 + bc) of one layer; LDS holds [bc][H + 1].
+template <int SIGN>
+__global__ void __launch_bounds__(1024)
+fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
+              unsigned long long radices, const float2* __restrict__ tw) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  // XCD-aware remap: hardware dispatches block b to XCD b % 8; give each XCD
+  // a contiguous range of column blocks so neighbours share its L2.
+  int blk = blockIdx.x;
+  if ((nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
+  const int c0 = blk * bc;
+  const int ncols = min(bc, W - c0);
+  const int pitch = H + 1;
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + c0;
+  const int total = ncols * H;
+  for (int i = tid; i < total; i += nthr) {
+    const int y = i / ncols, c = i - y * ncols;
+    lds[c * pitch + y] = base[size_t(y) * W + c];
+  }
+  __syncthreads();
+  fftLds<SIGN>(lds, pitch, ncols, H, radices, tw, tid, nthr);
+  for (int i = tid; i < total; i += nthr) {
+    const int y = i / ncols, c = i - y * ncols;
+    base[size_t(y) * W + c] = lds[c * pitch + y];
+  }
+}
+
+unsigned long long packRadices(const FftPlan& plan) {
+  unsigned long long r = 0;
+  for (int i = plan.nstages - 1; i >= 0; --i) r = (r << 4) | unsigned(plan.radix[i]);
+  return r;
+}
+
+// Threads such that every stage's butterflies fit in kMaxValues / R batches
+// per thread; 0 if more than 1024 would be needed.
+int threadsFor(const FftPlan& plan, int nfft) {
+  const int elements = nfft * plan.n;
+  for (int t = 64; t <= 1024; t *= 2) {
+    bool ok = true;
+    for (int s = 0; s < plan.nstages; ++s) {
+      const int R = plan.radix[s];
+      if (t * (kMaxValues / R) < elements / R) ok = false;
+    }
+    if (ok) return std::max(t, 64);
+  }
+  return 0;
+}
+
+}  // namespace
+
+hws_status makeFftPlan(hws_gridder* g, int n, FftPlan& plan) {
+  plan.n = n;
+  plan.nstages = 0;
+  int m = n;
+  // radix 8 first, then 4 / 2, then the odd factors
+  while (m % 8 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 8; m /= 8; }
+  while (m % 4 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 4; m /= 4; }
+  while (m % 2 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 2; m /= 2; }
+  for (int r : {3, 5, 7})
+    while (m % r == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = r; m /= r; }
+  if (m != 1)
+    return setLastError(g, HWS_ERR_UNSUPPORTED,
+                        "image dimension " + std::to_string(n) +
+                            " is not a product of 2, 3, 5 and 7");
+  if (n > 16384 || plan.nstages > 16)
+    return setLastError(g, HWS_ERR_UNSUPPORTED,
+                        "image dimension above 16384 is not supported");
+  std::vector<float2> tb(n), tf(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < n; ++k) {
+    const long double a = 2.0L * pi * (long double)k / (long double)n;
+    const float c = (float)cosl(a), s = (float)sinl(a);
+    tb[k] = make_float2(c, s);
+    tf[k] = make_float2(c, -s);
+  }
+  hws_status st;
+  if ((st = checkHip(g, hipMalloc(&plan.twBackward, n * sizeof(float2)), "hipMalloc twiddles")) != HWS_OK) return st;
+  if ((st = checkHip(g, hipMalloc(&plan.twForward, n * sizeof(float2)), "hipMalloc twiddles")) != HWS_OK) return st;
+  if ((st = checkHip(g, hipMemcpy(plan.twBackward, tb.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload")) != HWS_OK) return st;
+  return checkHip(g, hipMemcpy(plan.twForward, tf.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload");
+}
+
+void freeFftPlan(FftPlan& plan) {
+  if (plan.twBackward) (void)hipFree(plan.twBackward);
+  if (plan.twForward) (void)hipFree(plan.twForward);
+  plan.twBackward = plan.twForward = nullptr;
+}
+
+namespace {
+
+// Power-of-two launch shapes: rows take max(1, 4096/N) rows per block,
+// columns max(1, 8192/N) columns (at most 16); 16 values per thread.
+template <int SIGN, int N>
+void launchPow2(float2* data, int W, int H, int nbatch, const float2* twRow,
+                const float2* twCol, bool rows, hipStream_t stream) {
+  if (rows) {
+    constexpr int NFFT = (4096 / N) > 0 ? (4096 / N) : 1;
+    constexpr int E = NFFT * N;
+    constexpr int NT = (E / 16) < 64 ? 64 : (E / 16) > 1024 ? 1024 : (E / 16);
+    const int nfft = std::min(NFFT, H);
+    (void)nfft;
+    dim3 grid((H + NFFT - 1) / NFFT, nbatch);
+    hipLaunchKernelGGL((fftRowsPow2Kernel<SIGN, N, NT, NFFT>), grid, dim3(NT), 0, stream, data, H, twRow);
+  } else {
+    constexpr int NFFT0 = (8192 / N) > 0 ? (8192 / N) : 1;
+    constexpr int NFFT = NFFT0 > 16 ? 16 : NFFT0;
+    constexpr int E = NFFT * N;
+    constexpr int NT = (E / 16) < 64 ? 64 : (E / 16) > 1024 ? 1024 : (E / 16);
+    const int nColBlocks = (W + NFFT - 1) / NFFT;
+    dim3 grid(nColBlocks, nbatch);
+    hipLaunchKernelGGL((fftColsPow2Kernel<SIGN, N, NT, NFFT>), grid, dim3(NT), 0, stream, data, W, nColBlocks, twCol);
+  }
+}
+
+template <int SIGN>
+bool dispatchPow2(int n, float2* data, int W, int H, int nbatch, const float2* twRow,
+                  const float2* twCol, bool rows, hipStream_t stream) {
+  switch (n) {
+    case 64: launchPow2<SIGN, 64>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 128: launchPow2<SIGN, 128>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 256: launchPow2<SIGN, 256>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 512: launchPow2<SIGN, 512>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 1024: launchPow2<SIGN, 1024>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 2048: launchPow2<SIGN, 2048>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 4096: launchPow2<SIGN, 4096>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    case 8192: launchPow2<SIGN, 8192>(data, W, H, nbatch, twRow, twCol, rows, stream); return true;
+    default: return false;
+  }
+}
+
+}  // namespace
+
+hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign) {
+  if (nbatch <= 0) return HWS_OK;
+  const int W = g->W, H = g->H;
+  int maxLds = 65536;
+  (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, g->cfg.device);
+  const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
+  const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
+  bool rowsDone = false, colsDone = false;
+  if (sign > 0) {
+    rowsDone = dispatchPow2<+1>(W, data, W, H, nbatch, twRow, twCol, true, g->stream);
+    colsDone = dispatchPow2<+1>(H, data, W, H, nbatch, twRow, twCol, false, g->stream);
+  } else {
+    rowsDone = dispatchPow2<-1>(W, data, W, H, nbatch, twRow, twCol, true, g->stream);
+    colsDone = dispatchPow2<-1>(H, data, W, H, nbatch, twRow, twCol, false, g->stream);
+  }
+  if (rowsDone || colsDone) {
+    hws_status st = checkHip(g, hipGetLastError(), "fft pow2 launch");
+    if (st != HWS_OK) return st;
+  }
+  // rows: as many whole rows per block as fit 4096 values (at least one)
+  if (!rowsDone) {
+    int rows = std::max(1, 4096 / W);
+    rows = std::min(rows, H);
+    const int threads = threadsFor(g->rowPlan, rows);
+    const size_t lds = size_t(rows) * W * sizeof(float2);
+    if (threads == 0 || lds > size_t(maxLds))
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "row transform does not fit one workgroup");
+    dim3 grid((H + rows - 1) / rows, nbatch);
+    const unsigned long long radices = packRadices(g->rowPlan);
+    const float2* tw = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
+    if (sign > 0)
+      hipLaunchKernelGGL(fftRowsKernel<+1>, grid, dim3(threads), lds, g->stream, data, W, H, rows, radices, tw);
+    else
+      hipLaunchKernelGGL(fftRowsKernel<-1>, grid, dim3(threads), lds, g->stream, data, W, H, rows, radices, tw);
+    hws_status st = checkHip(g, hipGetLastError(), "fft rows launch");
+    if (st != HWS_OK) return st;
+  }
+  // columns: blocks of bc columns, [bc][H + 1] in LDS
+  if (!colsDone) {
+    int bc = std::max(1, 8192 / H);
+    bc = std::min(std::min(bc, 16), W);
+    while (bc > 1 && size_t(bc) * (H + 1) * sizeof(float2) > size_t(maxLds)) --bc;
+    int threads = threadsFor(g->colPlan, bc);
+    while (threads == 0 && bc > 1) threads = threadsFor(g->colPlan, --bc);
+    const size_t lds = size_t(bc) * size_t(H + 1) * sizeof(float2);
+    if (threads == 0 || lds > size_t(maxLds))
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "column transform does not fit one workgroup");
+    const int nColBlocks = (W + bc - 1) / bc;
+    dim3 grid(nColBlocks, nbatch);
+    const unsigned long long radices = packRadices(g->colPlan);
+    const float2* tw = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
+    if (sign > 0)
+      hipLaunchKernelGGL(fftColsKernel<+1>, grid, dim3(threads), lds, g->stream, data, W, H, bc, nColBlocks, radices, tw);
+    else
+      hipLaunchKernelGGL(fftColsKernel<-1>, grid, dim3(threads), lds, g->stream, data, W, H, bc, nColBlocks, radices, tw);
+    hws_status st = checkHip(g, hipGetLastError(), "fft cols launch");
+    if (st != HWS_OK) return st;
+  }
+  return HWS_OK;
+}
+
+}  // namespace hws

@@ -1,0 +1,1294 @@
+// Device kernels of the MI355X w-stacking core and their launchers.
+//
+// Invert (gridding/wstackinggridder.cpp:127-137, 323-336, 517-597, 691-715,
+// 773-827, 851-907) and predict (cpp:139-157, 599-689, 829-849, 909-969) in
+// two forms:
+//
+// Batched form (HWS_PRECISION_REFERENCE / _MIXED):
+//   1. binning: every (row, channel) sample is placed, in FP64 exactly as
+//      AddDataSample does, onto its w-layer and uv position; its 7x7
+//      footprint start selects a (layer, uv tile) bin. A counting sort
+//      (histogram, exclusive scan, atomic scatter) orders 8-byte records by
+//      bin and splits bins into chunks of at most kChunkItems samples.
+//   2. grid: persistent workgroups take chunks; each zeroes an LDS subgrid of
+//      (T + K - 1)^2 cells per image of the kind, deposits every sample's
+//      footprint with LDS float atomics (one lane per tap), then flushes the
+//      subgrid to the layer in HBM with global FP32 atomics, once.
+//   3. degrid mirrors it: the subgrid is gathered from the layer into LDS
+//      once per chunk, then one lane per sample sums its K*K taps.
+//   Image-domain passes loop over all layers of a pass per pixel: the
+//   w-correction phase is computed once per pixel and layer, and the layer
+//   sum is Kahan-compensated in FP32.
+//
+// Plain form (HWS_PRECISION_PLAIN_FP32): one launch per channel and image,
+// FP32 index and phase arithmetic, a global atomic per tap, one projection
+// launch per layer.
+#include "hws_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hws {
+
+hws_status setLastError(hws_gridder* g, hws_status s, const std::string& msg) {
+  if (g) g->err = msg;
+  return s;
+}
+
+hws_status checkHip(hws_gridder* g, hipError_t e, const char* what) {
+  if (e == hipSuccess) return HWS_OK;
+  const hws_status s =
+      (e == hipErrorOutOfMemory) ? HWS_ERR_OUT_OF_MEMORY : HWS_ERR_HIP;
+  return setLastError(g, s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+LayerMap layerMap(const hws_gridder* g, int kind) {
+  LayerMap m;
+  m.nW = g->nW;
+  m.minW = g->minW;
+  m.maxW = g->maxW;
+  m.isComplex = kind;
+  return m;
+}
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kThreads = 256;
+
+#define HWS_TRY(expr)                  \
+  do {                                 \
+    hws_status _s = (expr);            \
+    if (_s != HWS_OK) return _s;       \
+  } while (0)
+
+struct ImgList {
+  int n;
+  int idx[kMaxImages];
+};
+
+struct GeomArgs {
+  int W, H, K, OS, mid;
+  double dl, dm;
+  int tileSize, tilesX, tilesY;
+  int rotate;        // apply the l/m shift phase rotation to samples
+  double lShift, mShift;
+};
+
+// ----------------------------------------------------------------- fill
+__global__ void fillKernel(uint32_t* p, uint32_t value, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = value;
+}
+
+// ------------------------------------------------------------- binning
+// Layer index as WToLayer computes it (gridding/wstackinggridder.h:182-193);
+// returns -1 when the rounded value is negative or >= nW.
+__device__ __forceinline__ int layerOf(const LayerMap& m, double w) {
+  if (m.nW == 1) return 0;
+  double r;
+  if (m.isComplex)
+    r = round((w + m.maxW) * (m.nW - 1) / (m.maxW + m.maxW));
+  else
+    r = round((fabs(w) - m.minW) * (m.nW - 1) / (m.maxW - m.minW));
+  if (!(r >= 0.0) || r >= double(m.nW)) return -1;
+  return int(r);
+}
+
+// Placement of one sample: the arithmetic of AddDataSample
+// (gridding/wstackinggridder.cpp:528-564), FP64 throughout.
+struct Placement {
+  int layer;   // -1 off layer
+  int onGrid;  // 0 rejected by the acceptance window
+  int x, y;    // wrapped grid position of the footprint centre
+  int kx, ky;  // kernel table indices
+  int conj;
+};
+
+__device__ __forceinline__ Placement place(const GeomArgs& ga,
+                                           const LayerMap& lm, double u,
+                                           double v, double w) {
+  Placement p;
+  p.conj = 0;
+  if (w < 0.0 && !lm.isComplex) {
+    u = -u;
+    v = -v;
+    w = -w;
+    p.conj = 1;
+  }
+  p.layer = layerOf(lm, w);
+  const double xExact = u * ga.dl * double(ga.W);
+  const double yExact = v * ga.dm * double(ga.H);
+  p.onGrid = 0;
+  p.x = p.y = p.kx = p.ky = 0;
+  // bound before the int conversion (the reference converts unchecked)
+  if (!(fabs(xExact) < 1.0e9) || !(fabs(yExact) < 1.0e9)) return p;
+  int x = int(round(xExact));
+  int y = int(round(yExact));
+  int kx = int(round((xExact - double(x)) * ga.OS));
+  int ky = int(round((yExact - double(y)) * ga.OS));
+  kx = (kx + (ga.OS * 3) / 2) % ga.OS;
+  ky = (ky + (ga.OS * 3) / 2) % ga.OS;
+  if (x > -ga.W / 2 && y > -ga.H / 2 && x <= ga.W / 2 && y <= ga.H / 2) {
+    if (x < 0) x += ga.W;
+    if (y < 0) y += ga.H;
+    p.onGrid = 1;
+    p.x = x;
+    p.y = y;
+    p.kx = kx;
+    p.ky = ky;
+  }
+  return p;
+}
+
+__device__ __forceinline__ void waveCount(unsigned long long* counter,
+                                          bool pred) {
+  const unsigned long long mask = __ballot(pred);
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+// Pass 1: place every sample, write its record and bin (unsorted) and
+// histogram the bins. Samples are visited channel-major (row fastest) so the
+// lanes of a wave hit different baselines and the histogram atomics spread.
+__global__ void __launch_bounds__(kThreads)
+binCountKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
+               size_t nrow, int nchan, GeomArgs ga, LayerMap lm,
+               uint2* __restrict__ recTmp, uint32_t* __restrict__ binTmp,
+               uint32_t* __restrict__ counts,
+               unsigned long long* __restrict__ counters) {
+  const size_t n = nrow * size_t(nchan);
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool gridded = false, offGrid = false, offLayer = false;
+  if (i < n) {
+    const size_t ch = i / nrow;
+    const size_t row = i - ch * nrow;
+    const double lambda = wl[ch];
+    const double u = uvw[row * 3] / lambda, v = uvw[row * 3 + 1] / lambda,
+                 w = uvw[row * 3 + 2] / lambda;
+    const Placement p = place(ga, lm, u, v, w);
+    uint32_t bin = 0xFFFFFFFFu;
+    if (p.layer < 0) {
+      offLayer = true;
+    } else if (!p.onGrid) {
+      offGrid = true;
+    } else {
+      int xs = p.x - ga.mid, ys = p.y - ga.mid;
+      if (xs < 0) xs += ga.W;
+      if (ys < 0) ys += ga.H;
+      const int tx = xs / ga.tileSize, ty = ys / ga.tileSize;
+      const int xo = xs - tx * ga.tileSize, yo = ys - ty * ga.tileSize;
+      bin = uint32_t((size_t(p.layer) * ga.tilesY + ty) * ga.tilesX + tx);
+      const uint32_t idx = uint32_t(row * nchan + ch);
+      uint2 rec;
+      rec.x = idx | (uint32_t(p.conj) << 31);
+      rec.y = uint32_t(xo) | (uint32_t(yo) << 6) |
+              (uint32_t(p.ky * ga.OS + p.kx) << 12);
+      recTmp[i] = rec;
+      atomicAdd(&counts[bin], 1u);
+      gridded = true;
+    }
+    binTmp[i] = bin;
+  }
+  waveCount(&counters[0], gridded);
+  waveCount(&counters[1], offGrid);
+  waveCount(&counters[2], offLayer);
+}
+
+// Pass 3: scatter records to their sorted position.
+__global__ void __launch_bounds__(kThreads)
+binScatterKernel(size_t n, const uint2* __restrict__ recTmp,
+                 const uint32_t* __restrict__ binTmp,
+                 uint32_t* __restrict__ cursor, uint2* __restrict__ recSorted) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = binTmp[i];
+  if (b == 0xFFFFFFFFu) return;
+  const uint32_t pos = atomicAdd(&cursor[b], 1u);
+  recSorted[pos] = recTmp[i];
+}
+
+__global__ void chunkCountKernel(const uint32_t* __restrict__ counts,
+                                 size_t nbins, uint32_t* __restrict__ nchunks) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b < nbins) nchunks[b] = (counts[b] + kChunkItems - 1) / kChunkItems;
+}
+
+__global__ void chunkEmitKernel(const uint32_t* __restrict__ counts,
+                                const uint32_t* __restrict__ binStart,
+                                const uint32_t* __restrict__ chunkStart,
+                                size_t nbins, uint4* __restrict__ chunks) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const uint32_t c = counts[b];
+  uint32_t out = chunkStart[b];
+  for (uint32_t k = 0; k < c; k += kChunkItems, ++out)
+    chunks[out] = make_uint4(uint32_t(b), binStart[b] + k,
+                             min(kChunkItems, c - k), 0u);
+}
+
+// --------------------------------------------------------- exclusive scan
+constexpr int kScanItems = 8;
+constexpr int kScanBlock = kThreads * kScanItems;  // 2048 values per block
+
+__device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v,
+                                                       uint32_t* sh,
+                                                       uint32_t* total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    uint32_t t = (tid >= off) ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[tid];
+  *total = sh[kThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+// out[i] = sum_{j<i} in[j] within each block of kScanBlock values; block
+// totals to blockSums.
+__global__ void __launch_bounds__(kThreads)
+scanBlocksKernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                 size_t n, uint32_t* __restrict__ blockSums) {
+  __shared__ uint32_t sh[kThreads];
+  const size_t base = size_t(blockIdx.x) * kScanBlock + size_t(threadIdx.x) * kScanItems;
+  uint32_t vals[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t i = base + k;
+    vals[k] = (i < n) ? in[i] : 0u;
+    local += vals[k];
+  }
+  uint32_t total;
+  uint32_t run = blockExclusiveScan(local, sh, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t i = base + k;
+    if (i < n) out[i] = run;
+    run += vals[k];
+  }
+  if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+
+__global__ void scanAddKernel(uint32_t* __restrict__ out, size_t n,
+                              const uint32_t* __restrict__ blockOffsets) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += blockOffsets[i / kScanBlock];
+}
+
+// out[n] = total is written as well (out has n + 1 entries).
+__global__ void scanTotalKernel(const uint32_t* __restrict__ in,
+                                uint32_t* __restrict__ out, size_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    out[n] = (n > 0) ? out[n - 1] + in[n - 1] : 0u;
+}
+
+// ------------------------------------------------------------- grid (LDS)
+struct GridArgs {
+  GeomArgs ga;
+  ImgList imgs;      // images of this kind
+  int nImgTotal;     // images per sample in the visibility array
+  int passStart;     // first layer of the pass
+  int layerLo, layerHi;  // layers of the pass (absolute)
+  int nLayersPass;   // allocated layers per image block
+  int tilesPerLayer;
+  int sg;            // subgrid edge = tileSize + K - 1
+  size_t nchan;
+};
+
+__device__ __forceinline__ float2 rotateSample(float2 v, const GridArgs& a,
+                                               const double* __restrict__ uvw,
+                                               const double* __restrict__ wl,
+                                               uint32_t idx, double sign) {
+  // MSGridderBase::rotateVisibilities (gridding/msgridderbase.cpp:692-704)
+  // with shiftFactor = sign * 2 pi (u l_shift + v m_shift), FP64 argument.
+  const size_t row = idx / a.nchan, ch = idx - row * a.nchan;
+  const double shiftFactor = sign * 2.0 * kPi *
+                             (uvw[row * 3] * a.ga.lShift + uvw[row * 3 + 1] * a.ga.mShift);
+  const double rad = shiftFactor / wl[ch];
+  double s, c;
+  sincos(rad, &s, &c);
+  const float cf = float(c), sf = float(s);
+  return make_float2(v.x * cf - v.y * sf, v.x * sf + v.y * cf);
+}
+
+// Kernel tables into LDS: FP32 tables, or FP16 (hi, lo) pairs whose sum
+// restores the FP32 value to about 22 bits.
+__device__ __forceinline__ void loadKernelTable(float* kt, const float* __restrict__ ktab,
+                                                const __half2* __restrict__ khalf, int n) {
+  if (khalf) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const __half2 h = khalf[i];
+      kt[i] = __low2float(h) + __high2float(h);
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) kt[i] = ktab[i];
+  }
+}
+
+template <int NIMG>
+__global__ void __launch_bounds__(kThreads)
+gridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
+                const uint32_t* __restrict__ chunkStart,
+                const uint2* __restrict__ recs, const float2* __restrict__ vis,
+                const float* __restrict__ ktab, const __half2* __restrict__ khalf,
+                const double* __restrict__ uvw, const double* __restrict__ wl,
+                float2* __restrict__ layers) {
+  extern __shared__ float smem[];
+  const int K = a.ga.K, OS = a.ga.OS, sg = a.sg, sg2 = a.sg * a.sg;
+  float* kt = smem;                  // [OS][K]
+  float* planes = smem + ((OS * K + 3) & ~3);  // [NIMG*2][sg2]
+  loadKernelTable(kt, ktab, khalf, OS * K);
+  const uint32_t lo = chunkStart[size_t(a.layerLo) * a.tilesPerLayer];
+  const uint32_t hi = chunkStart[size_t(a.layerHi) * a.tilesPerLayer];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int taps = K * K;
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  for (uint32_t c = lo + blockIdx.x; c < hi; c += gridDim.x) {
+    const uint4 chunk = chunks[c];
+    const uint32_t bin = chunk.x, start = chunk.y, count = chunk.z;
+    const int layer = int(bin / uint32_t(a.tilesPerLayer));
+    const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
+    const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+    for (int i = threadIdx.x; i < NIMG * 2 * sg2; i += blockDim.x) planes[i] = 0.0f;
+    __syncthreads();
+    for (uint32_t it = wave; it < count; it += nwaves) {
+      const uint2 rec = recs[start + it];
+      const uint32_t idx = rec.x & 0x7FFFFFFFu;
+      const bool conj = (rec.x >> 31) != 0;
+      const int xo = int(rec.y & 63u), yo = int((rec.y >> 6) & 63u);
+      const int kk = int(rec.y >> 12);
+      const int ky = kk / OS, kx = kk - ky * OS;
+      float2 val[NIMG];
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+        if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, -1.0);
+        if (conj) v.y = -v.y;
+        val[gi] = v;
+      }
+      for (int t = lane; t < taps; t += 64) {
+        const int j = t / K, i = t - j * K;
+        // yKernelValue * xKernel[i] (gridding/wstackinggridder.cpp:574)
+        const float kv = kt[ky * K + j] * kt[kx * K + i];
+        const int cell = (yo + j) * sg + xo + i;
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          atomicAdd(&planes[(2 * gi) * sg2 + cell], val[gi].x * kv);
+          atomicAdd(&planes[(2 * gi + 1) * sg2 + cell], val[gi].y * kv);
+        }
+      }
+    }
+    __syncthreads();
+    const int x0 = tx * a.ga.tileSize, y0 = ty * a.ga.tileSize;
+    for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x) {
+      const int cy = cell / sg, cx = cell - cy * sg;
+      const int gx = (x0 + cx) % a.ga.W, gy = (y0 + cy) % a.ga.H;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        const float re = planes[(2 * gi) * sg2 + cell];
+        const float im = planes[(2 * gi + 1) * sg2 + cell];
+        if (re != 0.0f || im != 0.0f) {
+          float2* dst = layers +
+                        (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW +
+                        size_t(gy) * a.ga.W + gx;
+          unsafeAtomicAdd(&dst->x, re);
+          unsafeAtomicAdd(&dst->y, im);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NIMG>
+__global__ void __launch_bounds__(kThreads)
+degridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
+                  const uint32_t* __restrict__ chunkStart,
+                  const uint2* __restrict__ recs, const float* __restrict__ ktab,
+                  const __half2* __restrict__ khalf, const double* __restrict__ uvw,
+                  const double* __restrict__ wl, const float2* __restrict__ layers,
+                  float2* __restrict__ out) {
+  extern __shared__ float smem[];
+  const int K = a.ga.K, OS = a.ga.OS, sg = a.sg, sg2 = a.sg * a.sg;
+  float* kt = smem;
+  float2* sub = reinterpret_cast<float2*>(smem + ((OS * K + 3) & ~3));  // [NIMG][sg2]
+  loadKernelTable(kt, ktab, khalf, OS * K);
+  const uint32_t lo = chunkStart[size_t(a.layerLo) * a.tilesPerLayer];
+  const uint32_t hi = chunkStart[size_t(a.layerHi) * a.tilesPerLayer];
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  for (uint32_t c = lo + blockIdx.x; c < hi; c += gridDim.x) {
+    const uint4 chunk = chunks[c];
+    const uint32_t bin = chunk.x, start = chunk.y, count = chunk.z;
+    const int layer = int(bin / uint32_t(a.tilesPerLayer));
+    const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
+    const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+    const int x0 = tx * a.ga.tileSize, y0 = ty * a.ga.tileSize;
+    __syncthreads();  // previous chunk's readers are done with sub
+    for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x) {
+      const int cy = cell / sg, cx = cell - cy * sg;
+      const int gx = (x0 + cx) % a.ga.W, gy = (y0 + cy) % a.ga.H;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi)
+        sub[gi * sg2 + cell] =
+            layers[(size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW +
+                   size_t(gy) * a.ga.W + gx];
+    }
+    __syncthreads();
+    for (uint32_t it = threadIdx.x; it < count; it += blockDim.x) {
+      const uint2 rec = recs[start + it];
+      const uint32_t idx = rec.x & 0x7FFFFFFFu;
+      const bool conj = (rec.x >> 31) != 0;
+      const int xo = int(rec.y & 63u), yo = int((rec.y >> 6) & 63u);
+      const int kk = int(rec.y >> 12);
+      const int ky = kk / OS, kx = kk - ky * OS;
+      float2 acc[NIMG];
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) acc[gi] = make_float2(0.0f, 0.0f);
+      for (int j = 0; j < K; ++j) {
+        const float yk = kt[ky * K + j];
+        const int rowBase = (yo + j) * sg + xo;
+        for (int i = 0; i < K; ++i) {
+          // xKernel[i] * yKernelValue (gridding/wstackinggridder.cpp:668)
+          const float kv = kt[kx * K + i] * yk;
+#pragma unroll
+          for (int gi = 0; gi < NIMG; ++gi) {
+            const float2 gv = sub[gi * sg2 + rowBase + i];
+            acc[gi].x += gv.x * kv;
+            acc[gi].y += gv.y * kv;
+          }
+        }
+      }
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        // folded samples return the sample, others its conjugate (:681-684)
+        float2 v = conj ? acc[gi] : make_float2(acc[gi].x, -acc[gi].y);
+        if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, +1.0);
+        out[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]] = v;
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------- plain per-channel
+struct PlainArgs {
+  int W, H, K, OS, mid;
+  float dl, dm;
+  int nW;
+  float minW, maxW;
+  int isComplex;
+  int layerLo, layerHi, passStart, nLayersPass;
+  int img, nImgTotal;
+  int rotate;
+  float lShift, mShift;
+  int ch;
+  size_t nrow, nchan;
+};
+
+struct PlainPlacement {
+  int ok;
+  int layer, x, y, kx, ky, conj;
+  float rotRe, rotIm;
+};
+
+__device__ __forceinline__ PlainPlacement placePlain(const PlainArgs& a,
+                                                     const double* __restrict__ uvw,
+                                                     const double* __restrict__ wl,
+                                                     size_t r, float rotSign) {
+  PlainPlacement p;
+  p.ok = 0;
+  const float lambda = float(wl[a.ch]);
+  const float um = float(uvw[r * 3]), vm = float(uvw[r * 3 + 1]),
+              wm = float(uvw[r * 3 + 2]);
+  float u = um / lambda, v = vm / lambda, w = wm / lambda;
+  p.rotRe = 1.0f;
+  p.rotIm = 0.0f;
+  if (a.rotate) {
+    const float rad = rotSign * 2.0f * float(kPi) * (um * a.lShift + vm * a.mShift) / lambda;
+    sincosf(rad, &p.rotIm, &p.rotRe);
+  }
+  p.conj = 0;
+  if (w < 0.0f && !a.isComplex) {
+    u = -u;
+    v = -v;
+    w = -w;
+    p.conj = 1;
+  }
+  int layer = 0;
+  if (a.nW != 1) {
+    float lf = a.isComplex ? roundf((w + a.maxW) * (a.nW - 1) / (a.maxW + a.maxW))
+                           : roundf((fabsf(w) - a.minW) * (a.nW - 1) / (a.maxW - a.minW));
+    if (!(lf >= 0.0f) || lf >= float(a.nW)) return p;
+    layer = int(lf);
+  }
+  if (layer < a.layerLo || layer >= a.layerHi) return p;
+  const float xE = u * a.dl * float(a.W), yE = v * a.dm * float(a.H);
+  if (!(fabsf(xE) < 1.0e9f) || !(fabsf(yE) < 1.0e9f)) return p;
+  int x = int(roundf(xE)), y = int(roundf(yE));
+  int kx = int(roundf((xE - float(x)) * a.OS));
+  int ky = int(roundf((yE - float(y)) * a.OS));
+  kx = (kx + (a.OS * 3) / 2) % a.OS;
+  ky = (ky + (a.OS * 3) / 2) % a.OS;
+  if (!(x > -a.W / 2 && y > -a.H / 2 && x <= a.W / 2 && y <= a.H / 2)) return p;
+  if (x < 0) x += a.W;
+  if (y < 0) y += a.H;
+  p.ok = 1;
+  p.layer = layer;
+  p.x = x;
+  p.y = y;
+  p.kx = kx;
+  p.ky = ky;
+  return p;
+}
+
+__global__ void __launch_bounds__(kThreads)
+gridPlainKernel(PlainArgs a, const double* __restrict__ uvw,
+                const double* __restrict__ wl, const float2* __restrict__ vis,
+                const float* __restrict__ ktab, float2* __restrict__ layers) {
+  const size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= a.nrow) return;
+  const PlainPlacement p = placePlain(a, uvw, wl, r, -1.0f);
+  if (!p.ok) return;
+  float2 v = vis[(r * a.nchan + a.ch) * a.nImgTotal + a.img];
+  if (a.rotate) v = make_float2(v.x * p.rotRe - v.y * p.rotIm, v.x * p.rotIm + v.y * p.rotRe);
+  if (p.conj) v.y = -v.y;
+  const size_t HW = size_t(a.W) * a.H;
+  float2* layer = layers + (size_t(a.img) * a.nLayersPass + (p.layer - a.passStart)) * HW;
+  for (int j = 0; j < a.K; ++j) {
+    const float yk = ktab[p.ky * a.K + j];
+    int cy = p.y + j - a.mid;
+    if (cy < 0) cy += a.H;
+    if (cy >= a.H) cy -= a.H;
+    for (int i = 0; i < a.K; ++i) {
+      int cx = p.x + i - a.mid;
+      if (cx < 0) cx += a.W;
+      if (cx >= a.W) cx -= a.W;
+      const float kv = yk * ktab[p.kx * a.K + i];
+      float2* dst = layer + size_t(cy) * a.W + cx;
+      unsafeAtomicAdd(&dst->x, v.x * kv);
+      unsafeAtomicAdd(&dst->y, v.y * kv);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+degridPlainKernel(PlainArgs a, const double* __restrict__ uvw,
+                  const double* __restrict__ wl, const float* __restrict__ ktab,
+                  const float2* __restrict__ layers, float2* __restrict__ out) {
+  const size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= a.nrow) return;
+  const PlainPlacement p = placePlain(a, uvw, wl, r, 1.0f);
+  float2* dst = out + (r * a.nchan + a.ch) * a.nImgTotal + a.img;
+  if (!p.ok) return;  // stays NaN (filled before the first pass)
+  const size_t HW = size_t(a.W) * a.H;
+  const float2* layer = layers + (size_t(a.img) * a.nLayersPass + (p.layer - a.passStart)) * HW;
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int j = 0; j < a.K; ++j) {
+    const float yk = ktab[p.ky * a.K + j];
+    int cy = p.y + j - a.mid;
+    if (cy < 0) cy += a.H;
+    if (cy >= a.H) cy -= a.H;
+    for (int i = 0; i < a.K; ++i) {
+      int cx = p.x + i - a.mid;
+      if (cx < 0) cx += a.W;
+      if (cx >= a.W) cx -= a.W;
+      const float kv = ktab[p.kx * a.K + i] * yk;
+      const float2 gv = layer[size_t(cy) * a.W + cx];
+      acc.x += gv.x * kv;
+      acc.y += gv.y * kv;
+    }
+  }
+  float2 v = p.conj ? acc : make_float2(acc.x, -acc.y);
+  if (a.rotate) v = make_float2(v.x * p.rotRe - v.y * p.rotIm, v.x * p.rotIm + v.y * p.rotRe);
+  *dst = v;
+}
+
+// ------------------------------------------------------ image domain
+struct ImgArgs {
+  int W, H;
+  double dl, dm, lShift, mShift;
+  int precision;
+  LayerMap maps[2];  // real, complex
+  int isComplex[kMaxImages];
+  int passStart, nLayersInPass, nLayersPass;
+};
+
+// n' = sqrt(1 - l^2 - m^2) - 1 inside the horizon, else 0
+// (gridding/wstackinggridder.cpp:865-868, 925-928).
+__device__ __forceinline__ double nPrime(double l, double m) {
+  return (l * l + m * m < 1.0) ? sqrt(1.0 - l * l - m * m) - 1.0 : 0.0;
+}
+
+__device__ __forceinline__ void kahanAdd(float& sum, float& comp, float term) {
+  const float y = term - comp;
+  const float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
+
+// Invert: every layer of the pass onto the image, FFT-ordered pixel p.
+// projectOnImageAndCorrect (gridding/wstackinggridder.cpp:874-907) for all
+// layers of the pass at once.
+__global__ void __launch_bounds__(kThreads)
+projectKernel(ImgArgs a, const float2* __restrict__ layers,
+              float* __restrict__ sum, float* __restrict__ comp,
+              float* __restrict__ isum, float* __restrict__ icomp) {
+  const size_t HW = size_t(a.W) * a.H;
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  const int img = blockIdx.y;
+  const int cplx = a.isComplex[img];
+  const int x = int(p % a.W), y = int(p / a.W);
+  int ySrc = (a.H - y) + a.H / 2;
+  if (ySrc >= a.H) ySrc -= a.H;
+  int xSrc = x + a.W / 2;
+  if (xSrc >= a.W) xSrc -= a.W;
+  const double m = ((double)ySrc - (a.H / 2)) * a.dm + a.mShift;
+  const double l = ((a.W / 2) - (double)xSrc) * a.dl + a.lShift;
+  double np = nPrime(l, m);
+  if (a.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
+  const size_t o = size_t(img) * HW + size_t(ySrc) * a.W + xSrc;
+  float s = sum[o], c = comp[o];
+  float is = 0.0f, ic = 0.0f;
+  if (cplx) {
+    is = isum[o];
+    ic = icomp[o];
+  }
+  const LayerMap& map = a.maps[cplx];
+  const float2* src = layers + size_t(img) * a.nLayersPass * HW + p;
+  for (int li = 0; li < a.nLayersInPass; ++li) {
+    const float2 F = src[size_t(li) * HW];
+    const double w = layerToW(map, a.passStart + li);
+    float re, im;
+    if (a.precision == HWS_PRECISION_REFERENCE) {
+      double sn, cs;
+      sincos(-2.0 * kPi * w * np, &sn, &cs);
+      re = float(F.x * cs - F.y * sn);
+      im = float(F.x * sn + F.y * cs);
+    } else {
+      const double phi = w * np;
+      const float f = float(phi - rint(phi));
+      float sf, cf;
+      sincospif(2.0f * f, &sf, &cf);
+      // exp(-2 pi i phi): sin = -sf, cos = cf
+      re = F.x * cf + F.y * sf;
+      im = -F.x * sf + F.y * cf;
+    }
+    kahanAdd(s, c, re);
+    if (cplx) kahanAdd(is, ic, im);
+  }
+  sum[o] = s;
+  comp[o] = c;
+  if (cplx) {
+    isum[o] = is;
+    icomp[o] = ic;
+  }
+}
+
+// Plain form: one layer, FP32 phase from the float n' table.
+__global__ void __launch_bounds__(kThreads)
+projectPlainKernel(int W, int H, const float2* __restrict__ layer,
+                   const float* __restrict__ ntab, float w, int cplx,
+                   float* __restrict__ sum, float* __restrict__ isum) {
+  const size_t HW = size_t(W) * H;
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  const int x = int(p % W), y = int(p / W);
+  int ySrc = (H - y) + H / 2;
+  if (ySrc >= H) ySrc -= H;
+  int xSrc = x + W / 2;
+  if (xSrc >= W) xSrc -= W;
+  const size_t o = size_t(ySrc) * W + xSrc;
+  const float2 F = layer[p];
+  float s, c;
+  sincosf(-2.0f * float(kPi) * w * ntab[p], &s, &c);
+  sum[o] += F.x * c - F.y * s;
+  if (cplx) isum[o] += F.x * s + F.y * c;
+}
+
+// n' tables as the reference stores them (T = float): order 0 for invert
+// (cpp:851-872), 1 for prediction (cpp:909-932).
+__global__ void nTableKernel(int W, int H, double dl, double dm, double ls,
+                             double ms, int order, float* __restrict__ out) {
+  const size_t HW = size_t(W) * H;
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  const int x = int(p % W), y = int(p / W);
+  int xs, ys;
+  if (order == 0) {
+    ys = (H - y) + H / 2;
+    if (ys >= H) ys -= H;
+    xs = x + W / 2;
+    if (xs >= W) xs -= W;
+  } else {
+    ys = y + H / 2;
+    if (ys >= H) ys -= H;
+    xs = (W - x) + W / 2;
+    if (xs >= W) xs -= W;
+  }
+  const double m = ((double)ys - (H / 2)) * dm + ms;
+  const double l = ((W / 2) - (double)xs) * dl + ls;
+  out[p] = float(nPrime(l, m));
+}
+
+// FinalizeImage (cpp:698-715) + correctImageForKernel<false> for T=float
+// (cpp:773-827): image = ((sum - comp) * mult) / (Dx * Dy / OS^2).
+__global__ void __launch_bounds__(kThreads)
+finalizeKernel(int W, int H, const float* __restrict__ sum,
+               const float* __restrict__ comp, double mult,
+               const float* __restrict__ corrX, const float* __restrict__ corrY,
+               double norm, int correct, float* __restrict__ out) {
+  const size_t HW = size_t(W) * H;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= HW) return;
+  const size_t img = blockIdx.y;
+  const int x = int(i % W), y = int(i / W);
+  float v = sum[img * HW + i] - comp[img * HW + i];
+  v = float(double(v) * mult);
+  if (correct) {
+    const float xv = corrX[x >= W / 2 ? x - W / 2 : W / 2 - x];
+    const float yv = corrY[y >= H / 2 ? y - H / 2 : H / 2 - y];
+    v = float(double(v) / (double(xv * yv) * norm));
+  }
+  out[img * HW + i] = v;
+}
+
+// initializePrediction (cpp:829-849): horizon mask then divide by the
+// kernel correction.
+__global__ void __launch_bounds__(kThreads)
+initPredictionKernel(int W, int H, double dl, double dm, double ls, double ms,
+                     const float* __restrict__ in, float* __restrict__ model,
+                     const float* __restrict__ corrX,
+                     const float* __restrict__ corrY, double norm, int correct) {
+  const size_t HW = size_t(W) * H;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= HW) return;
+  const size_t img = blockIdx.y;
+  const int x = int(i % W), y = int(i / W);
+  const double m = ((double)y - (H / 2)) * dm + ms;
+  const double l = ((W / 2) - (double)x) * dl + ls;
+  float v = model[img * HW + i];
+  v = (isfinite(v) && l * l + m * m < 1.0) ? in[img * HW + i] : 0.0f;
+  if (correct) {
+    const float xv = corrX[x >= W / 2 ? x - W / 2 : W / 2 - x];
+    const float yv = corrY[y >= H / 2 ? y - H / 2 : H / 2 - y];
+    v = float(double(v) / (double(xv * yv) * norm));
+  }
+  model[img * HW + i] = v;
+}
+
+// copyImageToLayerAndInverseCorrect (cpp:934-969) for every layer of the
+// pass; layer-ordered pixel p reads model pixel (xDest, yDest).
+__global__ void __launch_bounds__(kThreads)
+imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
+                    const float* __restrict__ modelIm,
+                    float2* __restrict__ layers) {
+  const size_t HW = size_t(a.W) * a.H;
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  const int img = blockIdx.y;
+  const int cplx = a.isComplex[img];
+  const int x = int(p % a.W), y = int(p / a.W);
+  int yDest = y + a.H / 2;
+  if (yDest >= a.H) yDest -= a.H;
+  int xDest = (a.W - x) + a.W / 2;
+  if (xDest >= a.W) xDest -= a.W;
+  const double m = ((double)yDest - (a.H / 2)) * a.dm + a.mShift;
+  const double l = ((a.W / 2) - (double)xDest) * a.dl + a.lShift;
+  double np = nPrime(l, m);
+  if (a.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
+  const size_t src = size_t(img) * HW + size_t(yDest) * a.W + xDest;
+  const float re = modelRe[src];
+  const float imv = cplx ? -modelIm[src] : 0.0f;
+  const LayerMap& map = a.maps[cplx];
+  float2* dst = layers + size_t(img) * a.nLayersPass * HW + p;
+  for (int li = 0; li < a.nLayersInPass; ++li) {
+    const double w = layerToW(map, a.passStart + li);
+    float2 out;
+    if (a.precision == HWS_PRECISION_REFERENCE) {
+      double sn, cs;
+      sincos(2.0 * kPi * w * np, &sn, &cs);
+      if (cplx)
+        out = make_float2(float(re * cs + imv * sn), float(imv * cs - re * sn));
+      else
+        out = make_float2(float(re * cs), float(-re * sn));
+    } else {
+      const double phi = w * np;
+      const float f = float(phi - rint(phi));
+      float sn, cs;
+      sincospif(2.0f * f, &sn, &cs);
+      if (cplx)
+        out = make_float2(re * cs + imv * sn, imv * cs - re * sn);
+      else
+        out = make_float2(re * cs, -re * sn);
+    }
+    dst[size_t(li) * HW] = out;
+  }
+}
+
+// Plain form: one layer of one image from the float predict-order n' table.
+__global__ void __launch_bounds__(kThreads)
+imageToLayerPlainKernel(int W, int H, const float* __restrict__ modelRe,
+                        const float* __restrict__ modelIm,
+                        const float* __restrict__ ntab, float w, int cplx,
+                        float2* __restrict__ layer) {
+  const size_t HW = size_t(W) * H;
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  const int x = int(p % W), y = int(p / W);
+  int yDest = y + H / 2;
+  if (yDest >= H) yDest -= H;
+  int xDest = (W - x) + W / 2;
+  if (xDest >= W) xDest -= W;
+  const size_t src = size_t(yDest) * W + xDest;
+  float s, c;
+  sincosf(2.0f * float(kPi) * w * ntab[p], &s, &c);
+  const float re = modelRe[src];
+  if (cplx) {
+    const float imv = -modelIm[src];
+    layer[p] = make_float2(re * c + imv * s, imv * c - re * s);
+  } else {
+    layer[p] = make_float2(re * c, -re * s);
+  }
+}
+
+template <typename T>
+hws_status ensureAlloc(hws_gridder* g, T*& ptr, size_t& cap, size_t n,
+                       const char* what) {
+  if (cap >= n && ptr) return HWS_OK;
+  if (ptr) {
+    (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+  const hws_status st = checkHip(g, hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T)), what);
+  if (st == HWS_OK) cap = n;
+  return st;
+}
+
+unsigned blocksFor(size_t n, int threads = kThreads) {
+  return unsigned((n + threads - 1) / threads);
+}
+
+// Exclusive scan of n values into out[0..n] (out[n] = total); scratch holds
+// the block sums of every level (scanScratchSize(n) values).
+size_t scanScratchSize(size_t n) {
+  size_t total = 0;
+  while (n > size_t(kScanBlock)) {
+    const size_t nb = (n + kScanBlock - 1) / kScanBlock;
+    total += 2 * (nb + 1);
+    n = nb;
+  }
+  return total + 2 * 2 + 64;
+}
+
+hws_status exclusiveScan(hws_gridder* g, const uint32_t* in, uint32_t* out,
+                         size_t n, uint32_t* scratch) {
+  if (n == 0) {
+    hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, g->stream, out, 0u, size_t(1));
+    return checkHip(g, hipGetLastError(), "scan fill");
+  }
+  const size_t nblocks = (n + kScanBlock - 1) / kScanBlock;
+  uint32_t* sums = scratch;
+  uint32_t* offs = scratch + nblocks + 1;
+  hipLaunchKernelGGL(scanBlocksKernel, dim3(unsigned(nblocks)), dim3(kThreads), 0, g->stream, in, out, n, sums);
+  HWS_TRY(checkHip(g, hipGetLastError(), "scan blocks"));
+  if (nblocks > 1) {
+    HWS_TRY(exclusiveScan(g, sums, offs, nblocks, scratch + 2 * (nblocks + 1)));
+    hipLaunchKernelGGL(scanAddKernel, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream, out, n, offs);
+    HWS_TRY(checkHip(g, hipGetLastError(), "scan add"));
+  }
+  hipLaunchKernelGGL(scanTotalKernel, dim3(1), dim3(64), 0, g->stream, in, out, n);
+  return checkHip(g, hipGetLastError(), "scan total");
+}
+
+GeomArgs geomArgs(const hws_gridder* g) {
+  GeomArgs ga;
+  ga.W = g->W;
+  ga.H = g->H;
+  ga.K = g->K;
+  ga.OS = g->OS;
+  ga.mid = g->K / 2;
+  ga.dl = g->cfg.pixel_size_x;
+  ga.dm = g->cfg.pixel_size_y;
+  ga.tileSize = g->plan.tileSize;
+  ga.tilesX = g->plan.tilesX;
+  ga.tilesY = g->plan.tilesY;
+  ga.rotate = g->cfg.rotate_visibilities &&
+              (g->cfg.l_shift != 0.0 || g->cfg.m_shift != 0.0);
+  ga.lShift = g->cfg.l_shift;
+  ga.mShift = g->cfg.m_shift;
+  return ga;
+}
+
+ImgArgs imgArgs(const hws_gridder* g, int pass) {
+  ImgArgs a;
+  a.W = g->W;
+  a.H = g->H;
+  a.dl = g->cfg.pixel_size_x;
+  a.dm = g->cfg.pixel_size_y;
+  a.lShift = g->cfg.l_shift;
+  a.mShift = g->cfg.m_shift;
+  a.precision = g->cfg.precision;
+  a.maps[0] = layerMap(g, 0);
+  a.maps[1] = layerMap(g, 1);
+  for (int i = 0; i < kMaxImages; ++i) a.isComplex[i] = (i < g->nImg) ? g->isComplex[i] : 0;
+  a.passStart = passLayerStart(g, pass);
+  a.nLayersInPass = passLayerStart(g, pass + 1) - a.passStart;
+  a.nLayersPass = g->maxLayersPerPass;
+  return a;
+}
+
+size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
+  const int sg = g->plan.tileSize + g->K - 1;
+  const size_t kt = size_t((g->OS * g->K + 3) & ~3) * sizeof(float);
+  return kt + size_t(nimg) * sg * sg * (degrid ? sizeof(float2) : 2 * sizeof(float));
+}
+
+int persistentBlocks(const hws_gridder* g) {
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->cfg.device);
+  return std::max(1, cus) * 8;
+}
+
+}  // namespace
+
+hws_status launchFill(hws_gridder* g, void* ptr, uint32_t value, size_t n32) {
+  if (n32 == 0) return HWS_OK;
+  const unsigned blocks = unsigned(std::min<size_t>(blocksFor(n32), 65536));
+  hipLaunchKernelGGL(fillKernel, dim3(blocks), dim3(kThreads), 0, g->stream,
+                     static_cast<uint32_t*>(ptr), value, n32);
+  return checkHip(g, hipGetLastError(), "fill launch");
+}
+
+hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
+                     const double* dUvw, const double* dWl) {
+  Plan& pl = g->plan;
+  pl.valid = false;
+  const size_t n = nrow * nchan;
+  if (n >= (size_t(1) << 31))
+    return setLastError(g, HWS_ERR_UNSUPPORTED,
+                        "a plan holds at most 2^31 - 1 (row, channel) samples");
+  pl.nrow = nrow;
+  pl.nchan = nchan;
+  pl.uvw = dUvw;
+  pl.wavelengths = dWl;
+  pl.tileSize = int(g->cfg.tile_size ? g->cfg.tile_size : 32);
+  pl.tilesX = (g->W + pl.tileSize - 1) / pl.tileSize;
+  pl.tilesY = (g->H + pl.tileSize - 1) / pl.tileSize;
+  const size_t tpl = size_t(pl.tilesX) * pl.tilesY;
+  pl.nbins = tpl * size_t(g->nW);
+  if (pl.nbins >= 0xFFFFFFFFull)
+    return setLastError(g, HWS_ERR_UNSUPPORTED, "too many (layer, tile) bins");
+
+  HWS_TRY(ensureAlloc(g, pl.recTmp, pl.tmpCapacity, n, "plan records"));
+  HWS_TRY(ensureAlloc(g, pl.binTmp, pl.binTmpCapacity, n, "plan bins"));
+  HWS_TRY(ensureAlloc(g, pl.counts, pl.countsCapacity, pl.nbins + 1, "plan counts"));
+  HWS_TRY(ensureAlloc(g, pl.scanTmp, pl.scanCapacity, pl.nbins + 1, "plan scan"));
+  HWS_TRY(ensureAlloc(g, pl.scanScratch, pl.scanScratchCapacity, scanScratchSize(pl.nbins),
+                      "plan scan scratch"));
+  if (!pl.counters) {
+    HWS_TRY(checkHip(g, hipMalloc(&pl.counters, 4 * sizeof(unsigned long long)), "plan counters"));
+  }
+  HWS_TRY(launchFill(g, pl.counters, 0u, 8));
+
+  const GeomArgs ga = geomArgs(g);
+  for (int kind = 0; kind < 2; ++kind) {
+    KindPlan& kp = pl.kinds[kind];
+    kp.kind = kind;
+    kp.nImg = 0;
+    for (int i = 0; i < g->nImg; ++i)
+      if (int(g->isComplex[i]) == kind) kp.img[kp.nImg++] = i;
+    kp.active = kp.nImg > 0;
+    if (!kp.active) continue;
+    HWS_TRY(ensureAlloc(g, kp.recSorted, kp.capacity, n, "plan sorted records"));
+    HWS_TRY(ensureAlloc(g, kp.binStart, kp.binCapacity, pl.nbins + 1, "plan bin offsets"));
+    HWS_TRY(ensureAlloc(g, kp.chunkStart, kp.chunkStartCapacity, pl.nbins + 1, "plan chunk offsets"));
+    HWS_TRY(ensureAlloc(g, kp.chunks, kp.chunkCapacity, n / kChunkItems + pl.nbins + 1, "plan chunks"));
+
+    HWS_TRY(launchFill(g, pl.counts, 0u, pl.nbins + 1));
+    const LayerMap lm = layerMap(g, kind);
+    if (n > 0) {
+      hipLaunchKernelGGL(binCountKernel, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
+                         dUvw, dWl, nrow, int(nchan), ga, lm, pl.recTmp, pl.binTmp,
+                         pl.counts, pl.counters);
+      HWS_TRY(checkHip(g, hipGetLastError(), "bin count launch"));
+    }
+    HWS_TRY(exclusiveScan(g, pl.counts, kp.binStart, pl.nbins, pl.scanScratch));
+    hipLaunchKernelGGL(chunkCountKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
+                       pl.counts, pl.nbins, pl.scanTmp);
+    HWS_TRY(checkHip(g, hipGetLastError(), "chunk count launch"));
+    HWS_TRY(exclusiveScan(g, pl.scanTmp, kp.chunkStart, pl.nbins, pl.scanScratch));
+    hipLaunchKernelGGL(chunkEmitKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
+                       pl.counts, kp.binStart, kp.chunkStart, pl.nbins, kp.chunks);
+    HWS_TRY(checkHip(g, hipGetLastError(), "chunk emit launch"));
+    // cursor = bin offsets
+    HWS_TRY(checkHip(g, hipMemcpyAsync(pl.counts, kp.binStart, pl.nbins * sizeof(uint32_t),
+                                       hipMemcpyDeviceToDevice, g->stream), "cursor copy"));
+    if (n > 0) {
+      hipLaunchKernelGGL(binScatterKernel, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
+                         n, pl.recTmp, pl.binTmp, pl.counts, kp.recSorted);
+      HWS_TRY(checkHip(g, hipGetLastError(), "bin scatter launch"));
+    }
+  }
+  pl.valid = true;
+  return HWS_OK;
+}
+
+hws_status gridBinned(hws_gridder* g, const float* dVis, int pass) {
+  Plan& pl = g->plan;
+  if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
+  GridArgs a;
+  a.ga = geomArgs(g);
+  a.nImgTotal = g->nImg;
+  a.passStart = passLayerStart(g, pass);
+  a.layerLo = a.passStart;
+  a.layerHi = passLayerStart(g, pass + 1);
+  a.nLayersPass = g->maxLayersPerPass;
+  a.tilesPerLayer = pl.tilesX * pl.tilesY;
+  a.sg = pl.tileSize + g->K - 1;
+  a.nchan = pl.nchan;
+  const int blocks = persistentBlocks(g);
+  const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
+  for (int kind = 0; kind < 2; ++kind) {
+    const KindPlan& kp = pl.kinds[kind];
+    if (!kp.active) continue;
+    a.imgs.n = kp.nImg;
+    for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = kp.img[i];
+    const size_t lds = gridLdsBytes(g, kp.nImg, false);
+    const float2* vis = reinterpret_cast<const float2*>(dVis);
+    switch (kp.nImg) {
+#define HWS_GRID_CASE(N)                                                               \
+  case N:                                                                              \
+    hipLaunchKernelGGL(gridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream, \
+                       a, kp.chunks, kp.chunkStart, kp.recSorted, vis, g->dKernel, khalf, \
+                       pl.uvw, pl.wavelengths, reinterpret_cast<float2*>(g->dLayers)); \
+    break;
+      HWS_GRID_CASE(1)
+      HWS_GRID_CASE(2)
+      HWS_GRID_CASE(3)
+      HWS_GRID_CASE(4)
+#undef HWS_GRID_CASE
+    }
+    HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
+  }
+  return HWS_OK;
+}
+
+hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
+  Plan& pl = g->plan;
+  if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
+  GridArgs a;
+  a.ga = geomArgs(g);
+  a.nImgTotal = g->nImg;
+  a.passStart = passLayerStart(g, pass);
+  a.layerLo = a.passStart;
+  a.layerHi = passLayerStart(g, pass + 1);
+  a.nLayersPass = g->maxLayersPerPass;
+  a.tilesPerLayer = pl.tilesX * pl.tilesY;
+  a.sg = pl.tileSize + g->K - 1;
+  a.nchan = pl.nchan;
+  const int blocks = persistentBlocks(g);
+  const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
+  for (int kind = 0; kind < 2; ++kind) {
+    const KindPlan& kp = pl.kinds[kind];
+    if (!kp.active) continue;
+    a.imgs.n = kp.nImg;
+    for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = kp.img[i];
+    const size_t lds = gridLdsBytes(g, kp.nImg, true);
+    float2* out = reinterpret_cast<float2*>(dVisOut);
+    switch (kp.nImg) {
+#define HWS_DEGRID_CASE(N)                                                                \
+  case N:                                                                                 \
+    hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream, \
+                       a, kp.chunks, kp.chunkStart, kp.recSorted, g->dKernel, khalf, pl.uvw, \
+                       pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out); \
+    break;
+      HWS_DEGRID_CASE(1)
+      HWS_DEGRID_CASE(2)
+      HWS_DEGRID_CASE(3)
+      HWS_DEGRID_CASE(4)
+#undef HWS_DEGRID_CASE
+    }
+    HWS_TRY(checkHip(g, hipGetLastError(), "degrid launch"));
+  }
+  return HWS_OK;
+}
+
+namespace {
+PlainArgs plainArgs(const hws_gridder* g, int img, int pass, size_t nrow,
+                    size_t nchan) {
+  PlainArgs a;
+  a.W = g->W;
+  a.H = g->H;
+  a.K = g->K;
+  a.OS = g->OS;
+  a.mid = g->K / 2;
+  a.dl = float(g->cfg.pixel_size_x);
+  a.dm = float(g->cfg.pixel_size_y);
+  a.nW = g->nW;
+  a.minW = float(g->minW);
+  a.maxW = float(g->maxW);
+  a.isComplex = g->isComplex[img];
+  a.passStart = passLayerStart(g, pass);
+  a.layerLo = a.passStart;
+  a.layerHi = passLayerStart(g, pass + 1);
+  a.nLayersPass = g->maxLayersPerPass;
+  a.img = img;
+  a.nImgTotal = g->nImg;
+  a.rotate = g->cfg.rotate_visibilities &&
+             (g->cfg.l_shift != 0.0 || g->cfg.m_shift != 0.0);
+  a.lShift = float(g->cfg.l_shift);
+  a.mShift = float(g->cfg.m_shift);
+  a.ch = 0;
+  a.nrow = nrow;
+  a.nchan = nchan;
+  return a;
+}
+}  // namespace
+
+hws_status gridPlain(hws_gridder* g, size_t nrow, size_t nchan,
+                     const double* dUvw, const double* dWl, const float* dVis,
+                     int pass) {
+  if (nrow == 0) return HWS_OK;
+  for (int img = 0; img < g->nImg; ++img) {
+    PlainArgs a = plainArgs(g, img, pass, nrow, nchan);
+    for (size_t ch = 0; ch < nchan; ++ch) {
+      a.ch = int(ch);
+      hipLaunchKernelGGL(gridPlainKernel, dim3(blocksFor(nrow)), dim3(kThreads), 0, g->stream,
+                         a, dUvw, dWl, reinterpret_cast<const float2*>(dVis), g->dKernel,
+                         reinterpret_cast<float2*>(g->dLayers));
+      HWS_TRY(checkHip(g, hipGetLastError(), "plain grid launch"));
+    }
+  }
+  return HWS_OK;
+}
+
+hws_status degridPlain(hws_gridder* g, size_t nrow, size_t nchan,
+                       const double* dUvw, const double* dWl, float* dVisOut,
+                       int pass) {
+  if (nrow == 0) return HWS_OK;
+  for (int img = 0; img < g->nImg; ++img) {
+    PlainArgs a = plainArgs(g, img, pass, nrow, nchan);
+    for (size_t ch = 0; ch < nchan; ++ch) {
+      a.ch = int(ch);
+      hipLaunchKernelGGL(degridPlainKernel, dim3(blocksFor(nrow)), dim3(kThreads), 0, g->stream,
+                         a, dUvw, dWl, g->dKernel, reinterpret_cast<const float2*>(g->dLayers),
+                         reinterpret_cast<float2*>(dVisOut));
+      HWS_TRY(checkHip(g, hipGetLastError(), "plain degrid launch"));
+    }
+  }
+  return HWS_OK;
+}
+
+namespace {
+hws_status ensureNTable(hws_gridder* g, int order) {
+  const size_t HW = size_t(g->W) * g->H;
+  if (!g->dNTable)
+    HWS_TRY(checkHip(g, hipMalloc(&g->dNTable, HW * sizeof(float)), "n' table"));
+  if (g->nTableKind == order) return HWS_OK;
+  hipLaunchKernelGGL(nTableKernel, dim3(blocksFor(HW)), dim3(kThreads), 0, g->stream, g->W, g->H,
+                     g->cfg.pixel_size_x, g->cfg.pixel_size_y, g->cfg.l_shift, g->cfg.m_shift,
+                     order, g->dNTable);
+  HWS_TRY(checkHip(g, hipGetLastError(), "n' table launch"));
+  g->nTableKind = order;
+  return HWS_OK;
+}
+}  // namespace
+
+hws_status projectPass(hws_gridder* g, int pass) {
+  const size_t HW = size_t(g->W) * g->H;
+  const ImgArgs a = imgArgs(g, pass);
+  if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32) {
+    HWS_TRY(ensureNTable(g, 0));
+    for (int img = 0; img < g->nImg; ++img) {
+      const int kind = g->isComplex[img];
+      for (int li = 0; li < a.nLayersInPass; ++li) {
+        const float w = float(layerToW(a.maps[kind], a.passStart + li));
+        const float2* layer = reinterpret_cast<const float2*>(g->dLayers) +
+                              (size_t(img) * a.nLayersPass + li) * HW;
+        hipLaunchKernelGGL(projectPlainKernel, dim3(blocksFor(HW)), dim3(kThreads), 0, g->stream,
+                           g->W, g->H, layer, g->dNTable, w, kind, g->dSum + img * HW,
+                           g->dISum + img * HW);
+        HWS_TRY(checkHip(g, hipGetLastError(), "plain project launch"));
+      }
+    }
+    return HWS_OK;
+  }
+  hipLaunchKernelGGL(projectKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0, g->stream, a,
+                     reinterpret_cast<const float2*>(g->dLayers), g->dSum, g->dComp, g->dISum,
+                     g->dIComp);
+  return checkHip(g, hipGetLastError(), "project launch");
+}
+
+hws_status finalizeImages(hws_gridder* g, double mult, float* dReal,
+                          float* dImag) {
+  const size_t HW = size_t(g->W) * g->H;
+  const double norm = 1.0 / (double(g->OS) * double(g->OS));
+  const int correct = g->cfg.kernel_mode != HWS_KERNEL_NEAREST_NEIGHBOUR;
+  hipLaunchKernelGGL(finalizeKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0, g->stream,
+                     g->W, g->H, g->dSum, g->dComp, mult, g->dCorrXf, g->dCorrYf, norm, correct,
+                     dReal);
+  HWS_TRY(checkHip(g, hipGetLastError(), "finalize launch"));
+  if (dImag && g->anyComplex) {
+    hipLaunchKernelGGL(finalizeKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0, g->stream,
+                       g->W, g->H, g->dISum, g->dIComp, mult, g->dCorrXf, g->dCorrYf, norm,
+                       correct, dImag);
+    HWS_TRY(checkHip(g, hipGetLastError(), "finalize launch"));
+  }
+  return HWS_OK;
+}
+
+hws_status initPrediction(hws_gridder* g, const float* dReal,
+                          const float* dImag) {
+  const size_t HW = size_t(g->W) * g->H;
+  const double norm = 1.0 / (double(g->OS) * double(g->OS));
+  const int correct = g->cfg.kernel_mode != HWS_KERNEL_NEAREST_NEIGHBOUR;
+  hipLaunchKernelGGL(initPredictionKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0,
+                     g->stream, g->W, g->H, g->cfg.pixel_size_x, g->cfg.pixel_size_y,
+                     g->cfg.l_shift, g->cfg.m_shift, dReal, g->dModelRe, g->dCorrXf, g->dCorrYf,
+                     norm, correct);
+  HWS_TRY(checkHip(g, hipGetLastError(), "init prediction launch"));
+  if (g->anyComplex && dImag) {
+    hipLaunchKernelGGL(initPredictionKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0,
+                       g->stream, g->W, g->H, g->cfg.pixel_size_x, g->cfg.pixel_size_y,
+                       g->cfg.l_shift, g->cfg.m_shift, dImag, g->dModelIm, g->dCorrXf,
+                       g->dCorrYf, norm, correct);
+    HWS_TRY(checkHip(g, hipGetLastError(), "init prediction launch"));
+  }
+  return HWS_OK;
+}
+
+hws_status imageToLayers(hws_gridder* g, int pass) {
+  const size_t HW = size_t(g->W) * g->H;
+  const ImgArgs a = imgArgs(g, pass);
+  if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32) {
+    HWS_TRY(ensureNTable(g, 1));
+    for (int img = 0; img < g->nImg; ++img) {
+      const int kind = g->isComplex[img];
+      for (int li = 0; li < a.nLayersInPass; ++li) {
+        const float w = float(layerToW(a.maps[kind], a.passStart + li));
+        float2* layer = reinterpret_cast<float2*>(g->dLayers) + (size_t(img) * a.nLayersPass + li) * HW;
+        hipLaunchKernelGGL(imageToLayerPlainKernel, dim3(blocksFor(HW)), dim3(kThreads), 0,
+                           g->stream, g->W, g->H, g->dModelRe + img * HW,
+                           g->dModelIm ? g->dModelIm + img * HW : nullptr, g->dNTable, w, kind,
+                           layer);
+        HWS_TRY(checkHip(g, hipGetLastError(), "plain image-to-layer launch"));
+      }
+    }
+    return HWS_OK;
+  }
+  hipLaunchKernelGGL(imageToLayersKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0,
+                     g->stream, a, g->dModelRe, g->dModelIm,
+                     reinterpret_cast<float2*>(g->dLayers));
+  return checkHip(g, hipGetLastError(), "image-to-layers launch");
+}
+
+}  // namespace hws

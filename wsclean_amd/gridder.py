@@ -1,0 +1,230 @@
+"""Python face of the MI355X w-stacking core.
+
+``WStackingGridder`` mirrors the reference class WStackingGridder<T>
+(gridding/wstackinggridder.h:92-635) method for method, over host numpy
+arrays. ``DeviceGridder`` runs the batched one-call forms on torch tensors
+that already live on the GPU (used by the benchmark and the autograd op).
+Both call the C ABI of libhipwstacking.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import HwsConfig, HwsStats, KERNEL_MODES, PRECISIONS, check
+
+SPEED_OF_LIGHT = 299792458.0
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def _cfg(width, height, dl, dm, kernel_size, oversampling, kernel_mode, precision,
+         device, n_images, complex_mask, shift, rotate, tile_size):
+    mode = KERNEL_MODES[kernel_mode] if isinstance(kernel_mode, str) else int(kernel_mode)
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    return HwsConfig(width, height, dl, dm, kernel_size, oversampling, mode, prec, device,
+                     n_images, complex_mask, shift[0], shift[1], int(bool(rotate)),
+                     tile_size)
+
+
+class _Context:
+    def __init__(self, cfg: HwsConfig):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        st = self.L.hws_create(C.byref(cfg), C.byref(h))
+        check(st, None)
+        self.h = h
+        self.cfg = cfg
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hws_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, st):
+        check(st, self.h)
+
+
+def get_kernel(mode="kb", oversampling=63, size=7):
+    """WStackingGridder::GetKernel (gridding/wstackinggridder.cpp:387-423)."""
+    out = np.zeros(oversampling * size, np.float64)
+    L = _lib.load()
+    check(L.hws_get_kernel(KERNEL_MODES[mode], oversampling, size, _p(out, C.c_double)))
+    return out
+
+
+class WStackingGridder(_Context):
+    """Host-array mirror of WStackingGridder<float>.
+
+    width/height/pixel sizes as in the reference constructor
+    (gridding/wstackinggridder.h:120-122). ``n_images`` images sharing uvw can
+    be gridded at once; ``complex_images`` lists which of them are complex
+    (SetIsComplex). Visibility arrays are complex64 [row][chan][image]
+    (trailing image axis dropped when n_images == 1).
+    """
+
+    def __init__(self, width, height, pixel_size_x, pixel_size_y, kernel_size=7,
+                 oversampling=1023, *, kernel_mode="kb", precision="reference", device=0,
+                 n_images=1, complex_images=(), l_shift=0.0, m_shift=0.0,
+                 rotate_visibilities=False, tile_size=0):
+        mask = 0
+        for i in complex_images:
+            mask |= 1 << i
+        super().__init__(_cfg(width, height, pixel_size_x, pixel_size_y, kernel_size,
+                              oversampling, kernel_mode, precision, device, n_images, mask,
+                              (l_shift, m_shift), rotate_visibilities, tile_size))
+        self.width, self.height, self.n_images = width, height, n_images
+        self.complex_images = tuple(complex_images)
+        self.n_passes = 0
+
+    # ---- w-layers (gridding/wstackinggridder.h:158-267)
+    def prepare_w_layers(self, n_w_layers, max_mem, min_w, max_w):
+        n = C.c_size_t()
+        self._check(self.L.hws_prepare_w_layers(self.h, n_w_layers, max_mem, min_w, max_w,
+                                                C.byref(n)))
+        self.n_passes = n.value
+        return self.n_passes
+
+    def w_to_layer(self, w, is_complex=False):
+        r = self.L.hws_w_to_layer(self.h, w, int(is_complex))
+        return None if r == 2**64 - 1 else r
+
+    def layer_to_w(self, layer, is_complex=False):
+        return self.L.hws_layer_to_w(self.h, layer, int(is_complex))
+
+    def pass_layer_range(self, p):
+        a, b = C.c_size_t(), C.c_size_t()
+        self._check(self.L.hws_pass_layer_range(self.h, p, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # ---- inversion
+    def start_inversion_pass(self, p):
+        self._check(self.L.hws_start_inversion_pass(self.h, p))
+
+    def _vis(self, vis, nrow, nchan):
+        v = np.ascontiguousarray(vis, np.complex64).reshape(nrow, nchan, self.n_images)
+        return v.view(np.float32)
+
+    def add_data(self, uvw_m, wavelengths, vis):
+        uvw = np.ascontiguousarray(uvw_m, np.float64).reshape(-1, 3)
+        wl = np.ascontiguousarray(wavelengths, np.float64).reshape(-1)
+        v = self._vis(vis, uvw.shape[0], wl.shape[0])
+        self._check(self.L.hws_add_data(self.h, uvw.shape[0], wl.shape[0], _p(uvw, C.c_double),
+                                        _p(wl, C.c_double), _p(v, C.c_float)))
+
+    def add_samples(self, uvw_lambda, vis):
+        uvw = np.ascontiguousarray(uvw_lambda, np.float64).reshape(-1, 3)
+        v = self._vis(vis, uvw.shape[0], 1)
+        self._check(self.L.hws_add_samples(self.h, uvw.shape[0], _p(uvw, C.c_double),
+                                           _p(v, C.c_float)))
+
+    def finish_inversion_pass(self):
+        self._check(self.L.hws_finish_inversion_pass(self.h))
+
+    def finalize_image(self, multiplication_factor):
+        """Returns (real [n_images,H,W], imag or None)."""
+        n = self.n_images * self.width * self.height
+        re = np.zeros(n, np.float32)
+        im = np.zeros(n, np.float32) if self.complex_images else None
+        self._check(self.L.hws_finalize_image(self.h, multiplication_factor,
+                                              _p(re, C.c_float), _p(im, C.c_float)))
+        shape = (self.n_images, self.height, self.width)
+        return re.reshape(shape), (im.reshape(shape) if im is not None else None)
+
+    def get_gridded_uv_layer(self, layer, image=0):
+        out = np.zeros(self.width * self.height * 2, np.float32)
+        self._check(self.L.hws_get_gridded_uv_layer(self.h, image, layer, _p(out, C.c_float)))
+        return out.view(np.complex64).reshape(self.height, self.width)
+
+    # ---- prediction
+    def initialize_prediction(self, real, imag=None):
+        re = np.ascontiguousarray(real, np.float32).reshape(-1)
+        im = np.ascontiguousarray(imag, np.float32).reshape(-1) if imag is not None else None
+        self._check(self.L.hws_initialize_prediction(self.h, _p(re, C.c_float), _p(im, C.c_float)))
+
+    def start_prediction_pass(self, p):
+        self._check(self.L.hws_start_prediction_pass(self.h, p))
+
+    def sample_data(self, uvw_m, wavelengths):
+        uvw = np.ascontiguousarray(uvw_m, np.float64).reshape(-1, 3)
+        wl = np.ascontiguousarray(wavelengths, np.float64).reshape(-1)
+        out = np.zeros(uvw.shape[0] * wl.shape[0] * self.n_images * 2, np.float32)
+        self._check(self.L.hws_sample_data(self.h, uvw.shape[0], wl.shape[0], _p(uvw, C.c_double),
+                                           _p(wl, C.c_double), _p(out, C.c_float)))
+        return out.view(np.complex64).reshape(uvw.shape[0], wl.shape[0], self.n_images)
+
+    def sample_samples(self, uvw_lambda):
+        uvw = np.ascontiguousarray(uvw_lambda, np.float64).reshape(-1, 3)
+        out = np.zeros(uvw.shape[0] * self.n_images * 2, np.float32)
+        self._check(self.L.hws_sample_samples(self.h, uvw.shape[0], _p(uvw, C.c_double),
+                                              _p(out, C.c_float)))
+        return out.view(np.complex64).reshape(uvw.shape[0], self.n_images)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class DeviceGridder(_Context):
+    """Batched one-call invert/predict on torch CUDA (HIP) tensors.
+
+    uvw_m: float64 [nrow, 3]; wavelengths: float64 [nchan];
+    vis: complex64 [nrow, nchan, n_images]; images: float32 [n_images, H, W].
+    """
+
+    def __init__(self, width, height, pixel_size_x, pixel_size_y, kernel_size=7,
+                 oversampling=63, *, kernel_mode="kb", precision="mixed", device=0,
+                 n_images=1, complex_images=(), l_shift=0.0, m_shift=0.0,
+                 rotate_visibilities=True, tile_size=0, stream=None):
+        mask = 0
+        for i in complex_images:
+            mask |= 1 << i
+        super().__init__(_cfg(width, height, pixel_size_x, pixel_size_y, kernel_size,
+                              oversampling, kernel_mode, precision, device, n_images, mask,
+                              (l_shift, m_shift), rotate_visibilities, tile_size))
+        self.width, self.height, self.n_images = width, height, n_images
+        self.complex_images = tuple(complex_images)
+        self.device = device
+        self.n_passes = 0
+        self.nrow = self.nchan = 0
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream):
+        self._check(self.L.hws_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
+    def prepare_w_layers(self, n_w_layers, min_w, max_w, max_mem=0.0):
+        n = C.c_size_t()
+        self._check(self.L.hws_prepare_w_layers(self.h, n_w_layers, max_mem, min_w, max_w,
+                                                C.byref(n)))
+        self.n_passes = n.value
+        return self.n_passes
+
+    def plan(self, uvw_m, wavelengths):
+        self._uvw, self._wl = uvw_m, wavelengths  # keep alive
+        self.nrow, self.nchan = uvw_m.shape[0], wavelengths.shape[0]
+        self._check(self.L.hws_plan_device(self.h, self.nrow, self.nchan, _ptr(uvw_m),
+                                           _ptr(wavelengths)))
+
+    def invert(self, vis, multiplication_factor, real_out, imag_out=None):
+        self._check(self.L.hws_invert_planned_device(self.h, _ptr(vis), multiplication_factor,
+                                                     _ptr(real_out), _ptr(imag_out)))
+
+    def predict(self, real, imag, vis_out):
+        self._check(self.L.hws_predict_planned_device(self.h, _ptr(real), _ptr(imag),
+                                                      _ptr(vis_out)))
+
+    def synchronize(self):
+        self._check(self.L.hws_synchronize(self.h))
+
+    def stats(self):
+        s = HwsStats()
+        self._check(self.L.hws_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in HwsStats._fields_}
