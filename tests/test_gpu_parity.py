@@ -298,3 +298,32 @@ def test_gpu_predict_is_adjoint_of_invert(hws, precision):
         rhs = np.real(np.vdot(vis[:, :, k].astype(np.complex128).ravel(),
                               a[:, :, k].astype(np.complex128).ravel()))
         assert abs(lhs - rhs) < 1e-5 * abs(lhs), (k, lhs, rhs)
+
+
+@pytest.mark.parametrize("shape,precision", [((256, 256), "reference"), ((512, 256), "mixed"),
+                                             ((256, 256), "plain")])
+def test_blocked_layer_layout_sizes(hws, oracle, shape, precision):
+    # power-of-two sizes 256..4096 store layers with four rows interleaved and
+    # use the coalesced radix-16 transforms; layers read back in natural order
+    w, h = shape
+    uvw, wl, vis, dl = case(11, min(w, h), nrow=400, dl=0.003)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    img_o, lay_o = oracle_invert(oracle, w, h, dl, dl, uvw_l, vis.reshape(-1), 4, 0.0, maxw)
+    re, _, lay, _ = gpu_invert(hws, w, h, dl, dl, uvw, wl, vis, 4, 0.0, maxw, precision,
+                               layers=True)
+    for i in range(4):
+        if np.abs(lay_o[i]).sum() > 0:
+            assert rel(lay[0][i], lay_o[i]) < TOL_LAYER[precision]
+    assert rel(re[0], img_o) < TOL_IMG[precision]
+    rng = np.random.default_rng(5)
+    model = rng.standard_normal((h, w)).astype(np.float32)
+    v_o = oracle_predict(oracle, w, h, dl, dl, uvw_l, model, None, 4, 0.0, maxw)
+    g = hws.WStackingGridder(w, h, dl, dl, 7, 63, precision=precision)
+    g.prepare_w_layers(4, 0.0, 0.0, maxw)
+    g.initialize_prediction(model)
+    g.start_prediction_pass(0)
+    v = g.sample_data(uvw, wl).reshape(-1)
+    ok = ~np.isnan(v_o)
+    assert np.array_equal(np.isnan(v), ~ok)
+    assert rel(v[ok], v_o[ok]) < TOL_IMG[precision]

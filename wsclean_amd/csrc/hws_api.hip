@@ -336,6 +336,30 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   if ((st = checkHip(g, hipMemcpy(g->dCorrXf, cx.data(), cx.size() * sizeof(float), hipMemcpyHostToDevice), "correction upload")) != HWS_OK) return bail(st);
   if ((st = checkHip(g, hipMemcpy(g->dCorrYf, cy.data(), cy.size() * sizeof(float), hipMemcpyHostToDevice), "correction upload")) != HWS_OK) return bail(st);
 
+  // blocked layer layout where the coalesced radix-16 transforms cover the size
+  g->blocked = blockedLayoutSupported(g->W, g->H);
+  // uv tile edge of the binned forms: divides W and H, a multiple of 4, at
+  // least K - 1 (a footprint then spans at most two tiles per axis) and with
+  // T + K - 1 <= 64 (6-bit record offsets)
+  {
+    const int K = g->K;
+    const int want = c->tile_size ? int(c->tile_size) : 32;
+    int best = 0;
+    for (int t = 4; t + K - 1 <= 64 && t <= 64; t += 4)
+      if (g->W % t == 0 && g->H % t == 0 && t >= K - 1 && std::abs(t - want) <= std::abs(best - want))
+        best = t;
+    if (c->tile_size && best != want) {
+      gGlobalError = "tile_size must divide width and height, be a multiple of 4 and >= K - 1";
+      hws_destroy(g);
+      return HWS_ERR_INVALID_ARGUMENT;
+    }
+    if (best == 0 && c->precision != HWS_PRECISION_PLAIN_FP32) {
+      gGlobalError = "no uv tile size divides both image dimensions";
+      hws_destroy(g);
+      return HWS_ERR_UNSUPPORTED;
+    }
+    g->tileSize = best ? best : 32;
+  }
   if ((st = makeFftPlan(g, g->W, g->rowPlan)) != HWS_OK) return bail(st);
   if ((st = makeFftPlan(g, g->H, g->colPlan)) != HWS_OK) return bail(st);
   *out = g;
@@ -366,17 +390,13 @@ void hws_destroy(hws_gridder* g) {
   freeFftPlan(g->rowPlan);
   freeFftPlan(g->colPlan);
   Plan& pl = g->plan;
-  freeDev(pl.recTmp);
-  freeDev(pl.binTmp);
-  freeDev(pl.counts);
   freeDev(pl.scanTmp);
   freeDev(pl.scanScratch);
   freeDev(pl.counters);
   for (auto& kp : pl.kinds) {
     freeDev(kp.recSorted);
+    freeDev(kp.counts);
     freeDev(kp.binStart);
-    freeDev(kp.chunkStart);
-    freeDev(kp.chunks);
   }
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
@@ -498,7 +518,7 @@ static hws_status addRows(hws_gridder* g, size_t nrow, size_t nchan,
     HWS_TRY(gridPlain(g, nrow, nchan, g->dStageUvw, g->dStageWl, g->dStageVis, g->curPass));
   } else {
     HWS_TRY(buildPlan(g, nrow, nchan, g->dStageUvw, g->dStageWl));
-    HWS_TRY(gridBinned(g, g->dStageVis, g->curPass));
+    HWS_TRY(gridBinned(g, g->dStageVis, g->curPass, true));
   }
   return checkHip(g, hipStreamSynchronize(g->stream), "add data");
 }
@@ -633,8 +653,20 @@ hws_status hws_get_gridded_uv_layer(hws_gridder* g, size_t image, size_t layer,
   (void)hipSetDevice(g->cfg.device);
   const size_t HW = size_t(g->W) * g->H;
   const float2* src = g->dLayers + (image * g->maxLayersPerPass + layer) * HW;
-  HWS_TRY(checkHip(g, hipMemcpyAsync(out, src, HW * sizeof(float2), hipMemcpyDeviceToHost, g->stream), "layer download"));
-  return checkHip(g, hipStreamSynchronize(g->stream), "layer download");
+  if (!g->blocked) {
+    HWS_TRY(checkHip(g, hipMemcpyAsync(out, src, HW * sizeof(float2), hipMemcpyDeviceToHost, g->stream), "layer download"));
+    return checkHip(g, hipStreamSynchronize(g->stream), "layer download");
+  }
+  std::vector<float2> tmp(HW);
+  HWS_TRY(checkHip(g, hipMemcpyAsync(tmp.data(), src, HW * sizeof(float2), hipMemcpyDeviceToHost, g->stream), "layer download"));
+  HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "layer download"));
+  for (int y = 0; y < g->H; ++y)
+    for (int x = 0; x < g->W; ++x) {
+      const float2 v = tmp[cellIndex(x, y, g->W, true)];
+      out[2 * (size_t(y) * g->W + x)] = v.x;
+      out[2 * (size_t(y) * g->W + x) + 1] = v.y;
+    }
+  return HWS_OK;
 }
 
 hws_status hws_plan_device(hws_gridder* g, size_t nrow, size_t nchan,
@@ -672,12 +704,15 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
   HWS_TRY(zeroAccumulators(g));
   float ms[3] = {0, 0, 0};
   for (int pass = 0; pass < g->nPasses; ++pass) {
-    HWS_TRY(zeroLayers(g));
-    g->layersZero = false;
-    if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32)
+    if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32) {
+      HWS_TRY(zeroLayers(g));
+      g->layersZero = false;
       HWS_TRY(gridPlain(g, pl.nrow, pl.nchan, pl.uvw, pl.wavelengths, d_vis, pass));
-    else
-      HWS_TRY(gridBinned(g, d_vis, pass));
+    } else {
+      // every tile of the pass's layers is written, empty ones as zeros
+      g->layersZero = false;
+      HWS_TRY(gridBinned(g, d_vis, pass, false));
+    }
     HWS_TRY(record(g, 3));
     HWS_TRY(fftPassLayers(g, pass, +1));
     HWS_TRY(record(g, 4));
@@ -792,11 +827,12 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
     s.ms_total = total;
   }
   if (g->plan.counters) {
-    unsigned long long c[4] = {0, 0, 0, 0};
+    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
     HWS_TRY(checkHip(g, hipMemcpy(c, g->plan.counters, sizeof(c), hipMemcpyDeviceToHost), "counters"));
-    s.samples_gridded = c[0];
-    s.samples_off_grid = c[1];
-    s.samples_off_layer = c[2];
+    // summed over the image kinds present (each kind places every sample)
+    s.samples_gridded = c[0] + c[3];
+    s.samples_off_grid = c[1] + c[4];
+    s.samples_off_layer = c[2] + c[5];
   }
   *out = s;
   return HWS_OK;

@@ -229,6 +229,137 @@ __device__ __forceinline__ void pow2Fft(float2* buf, int pitch, int nfft,
   }
 }
 
+// ---- blocked layer layout: element (x, y) of a W x H layer is stored at
+// ((y >> 2) * W + x) * 4 + (y & 3), i.e. groups of four rows interleaved.
+// A row block (four rows) is then one contiguous run of 4W values and the
+// four columns x0..x0+3 form 128-byte segments, so both passes of the 2-D
+// transform read and write HBM fully coalesced. Each pass keeps four
+// transforms of length N (N/4 threads, 16 values per thread) in flight:
+// stage 0 reads HBM into registers, stages exchange through one padded LDS
+// buffer, the last stage writes HBM. Radix 16 stages, then 8, 4 or 2.
+
+__device__ __forceinline__ int padIdx(int e) { return e + (e >> 6); }
+
+template <int SIGN>
+struct Butterfly<SIGN, 16> {
+  __device__ __forceinline__ static void run(float2* v) {
+    // 4 x 4: DFT4 over stride-4 inputs, twiddle w16^(n1 k2), DFT4 again
+    float2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = v[n2 + 4 * n1];
+      Butterfly<SIGN, 4>::run(a[n2]);  // a[n2][k1]
+    }
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float h = 0.70710678118654752440f;
+    // twiddle a[n2][k1] *= w16^(n2 k1), w16 = exp(SIGN 2 pi i / 16)
+    const float2 w1 = make_float2(c1, SIGN * s1), w2 = make_float2(h, SIGN * h),
+                 w3 = make_float2(s1, SIGN * c1), w4 = make_float2(0.0f, float(SIGN)),
+                 w6 = make_float2(-h, SIGN * h), w9 = make_float2(-c1, -SIGN * s1);
+    a[1][1] = cmul(a[1][1], w1);
+    a[1][2] = cmul(a[1][2], w2);
+    a[1][3] = cmul(a[1][3], w3);
+    a[2][1] = cmul(a[2][1], w2);
+    a[2][2] = cmul(a[2][2], w4);
+    a[2][3] = cmul(a[2][3], w6);
+    a[3][1] = cmul(a[3][1], w3);
+    a[3][2] = cmul(a[3][2], w6);
+    a[3][3] = cmul(a[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Butterfly<SIGN, 4>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+  }
+};
+
+template <bool ROWS>
+__device__ __forceinline__ size_t blockedAddr(int pos, int f, int W) {
+  // ROWS: pos = x, f = row in block; COLS: pos = y, f = column in block
+  if (ROWS) return size_t(pos) * 4 + f;
+  return (size_t(pos >> 2) * W + f) * 4 + (pos & 3);
+}
+
+template <int SIGN, int R, int N, int NS, bool ROWS>
+__device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g, int W,
+                                             const float2* __restrict__ tw, int tid) {
+  constexpr int B = N / R;
+  constexpr int T = N / 4;
+  constexpr int PER = 16 / R;
+  constexpr int TWSTEP = N / (NS * R);
+  constexpr bool FIRST = NS == 1;
+  constexpr bool LAST = NS * R == N;
+  float2 v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int bt = tid + q * T;
+    const int f = bt & 3, b = bt >> 2;
+    const int jm = b % NS;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int pos = b + r * B;
+      float2 x = FIRST ? g[blockedAddr<ROWS>(pos, f, W)] : lds[padIdx(pos * 4 + f)];
+      if (!FIRST && r != 0) x = cmul(x, tw[jm * r * TWSTEP]);
+      v[q][r] = x;
+    }
+  }
+  if (!FIRST) __syncthreads();  // every read of this stage precedes its writes
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int bt = tid + q * T;
+    const int f = bt & 3, b = bt >> 2;
+    const int jm = b % NS;
+    Butterfly<SIGN, R>::run(v[q]);
+    const int base = (b / NS) * NS * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int pos = base + r * NS;
+      if (LAST)
+        g[blockedAddr<ROWS>(pos, f, W)] = v[q][r];
+      else
+        lds[padIdx(pos * 4 + f)] = v[q][r];
+    }
+  }
+  if (!LAST) __syncthreads();
+}
+
+template <int SIGN, int N, bool ROWS, int NS = 1>
+__device__ __forceinline__ void blockedFft(float2* lds, float2* __restrict__ g, int W,
+                                           const float2* __restrict__ tw, int tid) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = (REM % 16 == 0) ? 16 : REM;  // 16s, then the remaining 8, 4 or 2
+    static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix");
+    blockedStage<SIGN, R, N, NS, ROWS>(lds, g, W, tw, tid);
+    blockedFft<SIGN, N, ROWS, NS * R>(lds, g, W, tw, tid);
+  }
+}
+
+// grid (H / 4, nbatch): block = row block yb of layer blockIdx.y
+template <int SIGN, int N>
+__global__ void __launch_bounds__(N / 4)
+fftRowsBlockedKernel(float2* __restrict__ data, int H, const float2* __restrict__ tw) {
+  __shared__ float2 lds[4 * N + (4 * N) / 64];
+  float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
+  blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x);
+}
+
+// grid (W / 4, nbatch): block = columns 4*cb .. 4*cb+3 of layer blockIdx.y;
+// column blocks sharing an XCD are contiguous
+template <int SIGN, int N>
+__global__ void __launch_bounds__(N / 4)
+fftColsBlockedKernel(float2* __restrict__ data, int W, const float2* __restrict__ tw) {
+  __shared__ float2 lds[4 * N + (4 * N) / 64];
+  const int nb = gridDim.x;
+  int cb = blockIdx.x;
+  if ((nb & 7) == 0) cb = (cb & 7) * (nb >> 3) + (cb >> 3);
+  float2* g = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + size_t(cb) * 16;
+  blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x);
+}
+
 // rows of length N, NFFT rows per block (the last block may hold fewer)
 template <int SIGN, int N, int NT, int NFFT>
 __global__ void __launch_bounds__(NT)
@@ -423,9 +554,52 @@ bool dispatchPow2(int n, float2* data, int W, int H, int nbatch, const float2* t
 
 }  // namespace
 
+namespace {
+template <int SIGN>
+bool launchBlocked(int n, bool rows, float2* data, int W, int H, int nbatch,
+                   const float2* tw, hipStream_t stream) {
+#define HWS_BLOCKED_CASE(NN)                                                            \
+  case NN:                                                                              \
+    if (rows)                                                                           \
+      hipLaunchKernelGGL((fftRowsBlockedKernel<SIGN, NN>), dim3(H / 4, nbatch),         \
+                         dim3(NN / 4), 0, stream, data, H, tw);                         \
+    else                                                                                \
+      hipLaunchKernelGGL((fftColsBlockedKernel<SIGN, NN>), dim3(W / 4, nbatch),         \
+                         dim3(NN / 4), 0, stream, data, W, tw);                         \
+    return true;
+  switch (n) {
+    HWS_BLOCKED_CASE(256)
+    HWS_BLOCKED_CASE(512)
+    HWS_BLOCKED_CASE(1024)
+    HWS_BLOCKED_CASE(2048)
+    HWS_BLOCKED_CASE(4096)
+    default: return false;
+  }
+#undef HWS_BLOCKED_CASE
+}
+}  // namespace
+
+bool blockedLayoutSupported(int W, int H) {
+  auto ok = [](int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0; };
+  return ok(W) && ok(H);
+}
+
 hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign) {
   if (nbatch <= 0) return HWS_OK;
   const int W = g->W, H = g->H;
+  if (g->blocked) {
+    const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
+    const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
+    bool ok;
+    if (sign > 0)
+      ok = launchBlocked<+1>(W, true, data, W, H, nbatch, twRow, g->stream) &&
+           launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);
+    else
+      ok = launchBlocked<-1>(W, true, data, W, H, nbatch, twRow, g->stream) &&
+           launchBlocked<-1>(H, false, data, W, H, nbatch, twCol, g->stream);
+    if (!ok) return setLastError(g, HWS_ERR_UNSUPPORTED, "blocked layout size not supported");
+    return checkHip(g, hipGetLastError(), "fft blocked launch");
+  }
   int maxLds = 65536;
   (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, g->cfg.device);
   const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;

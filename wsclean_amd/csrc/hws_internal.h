@@ -19,7 +19,6 @@ constexpr int kMaxImages = 4;
 constexpr int kMaxFftStages = 24;
 constexpr int kMaxKernelSize = 15;
 constexpr int kMaxOversampling = 1023;
-constexpr uint32_t kChunkItems = 1024;  // max samples per binned work item
 
 struct Error {
   hws_status status;
@@ -57,22 +56,18 @@ __host__ __device__ inline double layerToW(const LayerMap& m, int layer) {
   return layer * (m.maxW - m.minW) / (m.nW - 1) + m.minW;
 }
 
-// Binned samples of one image kind: sorted records + per-bin offsets +
-// chunk list (bin, first record, count).
+// Binned samples of one image kind: one record per (sample, touched uv
+// tile), sorted by (layer, tile) bin, with per-bin offsets.
 struct KindPlan {
   bool active = false;
   int kind = 0;  // 0 real, 1 complex
   int nImg = 0;
   int img[kMaxImages] = {};
   uint64_t nItems = 0;
-  uint2* recSorted = nullptr;   // [capacity]
-  uint32_t* binStart = nullptr;  // [nbins + 1]
-  uint32_t* chunkStart = nullptr;  // [nbins + 1]
-  uint4* chunks = nullptr;         // [capacity chunks]
-  size_t chunkCapacity = 0;
-  size_t chunkStartCapacity = 0;
-  size_t capacity = 0;
-  size_t binCapacity = 0;
+  uint2* recSorted = nullptr;    // records ordered by bin [capacity]
+  uint32_t* counts = nullptr;    // histogram, then scatter cursor [nbins + 1]
+  uint32_t* binStart = nullptr;  // exclusive scan of counts [nbins + 1]
+  size_t countsCapacity = 0, capacity = 0, binCapacity = 0;
 };
 
 struct Plan {
@@ -84,14 +79,11 @@ struct Plan {
   size_t nbins = 0;
   KindPlan kinds[2];
   // temporaries
-  uint2* recTmp = nullptr;
-  uint32_t* binTmp = nullptr;
-  uint32_t* counts = nullptr;
   uint32_t* scanTmp = nullptr;
   uint32_t* scanScratch = nullptr;
-  size_t tmpCapacity = 0, binTmpCapacity = 0, countsCapacity = 0, scanCapacity = 0;
-  size_t scanScratchCapacity = 0;
-  unsigned long long* counters = nullptr;  // [4]: gridded, off grid, off layer, items
+  size_t scanCapacity = 0, scanScratchCapacity = 0;
+  // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}]
+  unsigned long long* counters = nullptr;
 };
 
 }  // namespace hws
@@ -133,6 +125,8 @@ struct hws_gridder {
 
   // FFT
   hws::FftPlan rowPlan, colPlan;
+  // layer layout: blocked (four rows interleaved, see cellIndex) or natural
+  bool blocked = false;
 
   // binning plan
   hws::Plan plan;
@@ -148,6 +142,7 @@ struct hws_gridder {
   hipEvent_t ev[8] = {};
   hws_stats stats{};
   int lastOp = 0;  // 1 invert, 2 predict (device one-call forms)
+  int tileSize = 32;  // uv tile edge of the binned forms (divides W and H)
 };
 
 namespace hws {
@@ -159,7 +154,7 @@ hws_status checkHip(hws_gridder* g, hipError_t e, const char* what);
 hws_status launchFill(hws_gridder* g, void* ptr, uint32_t value, size_t n32);
 hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl);
-hws_status gridBinned(hws_gridder* g, const float* dVis, int pass);
+hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate);
 hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass);
 hws_status gridPlain(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl, const float* dVis,
@@ -179,6 +174,28 @@ hws_status makeFftPlan(hws_gridder* g, int n, FftPlan& plan);
 void freeFftPlan(FftPlan& plan);
 // Unnormalised 2-D transform of nbatch H x W complex arrays, in place.
 hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign);
+
+// Storage index of layer cell (x, y): natural row-major, or blocked with
+// groups of four rows interleaved element by element.
+__host__ __device__ inline size_t cellIndex(int x, int y, int W, bool blocked) {
+  return blocked ? ((size_t(y >> 2) * size_t(W) + size_t(x)) << 2) + size_t(y & 3)
+                 : size_t(y) * size_t(W) + size_t(x);
+}
+
+// Inverse of cellIndex for storage index p.
+__host__ __device__ inline void cellOf(size_t p, int W, bool blocked, int& x, int& y) {
+  if (blocked) {
+    const size_t rowBlock = p / (size_t(4) * W);
+    const int rem = int(p - rowBlock * size_t(4) * W);
+    x = rem >> 2;
+    y = int(rowBlock) * 4 + (rem & 3);
+  } else {
+    y = int(p / size_t(W));
+    x = int(p - size_t(y) * W);
+  }
+}
+
+bool blockedLayoutSupported(int W, int H);
 
 inline int passLayerStart(const hws_gridder* g, int pass) {
   return int((size_t(g->nW) * size_t(pass)) / size_t(g->nPasses));

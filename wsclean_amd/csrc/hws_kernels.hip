@@ -6,16 +6,20 @@
 //
 // Batched form (HWS_PRECISION_REFERENCE / _MIXED):
 //   1. binning: every (row, channel) sample is placed, in FP64 exactly as
-//      AddDataSample does, onto its w-layer and uv position; its 7x7
-//      footprint start selects a (layer, uv tile) bin. A counting sort
-//      (histogram, exclusive scan, atomic scatter) orders 8-byte records by
-//      bin and splits bins into chunks of at most kChunkItems samples.
-//   2. grid: persistent workgroups take chunks; each zeroes an LDS subgrid of
-//      (T + K - 1)^2 cells per image of the kind, deposits every sample's
-//      footprint with LDS float atomics (one lane per tap), then flushes the
-//      subgrid to the layer in HBM with global FP32 atomics, once.
-//   3. degrid mirrors it: the subgrid is gathered from the layer into LDS
-//      once per chunk, then one lane per sample sums its K*K taps.
+//      AddDataSample does, onto its w-layer and uv position. Its K x K
+//      footprint touches one to four T x T uv tiles; a record (8 bytes) is
+//      emitted per touched (layer, tile) bin, the one holding the footprint
+//      start marked primary. A counting sort (histogram, exclusive scan,
+//      scatter, with one atomic per run of same-bin lanes) orders records by
+//      bin.
+//   2. grid (owner computes): a workgroup owns a (layer, tile), deposits all
+//      its records into an LDS tile (LDS float atomics, one lane per tap,
+//      register pre-accumulation while the footprint stays put) and writes the
+//      tile to the layer once with plain coalesced stores; empty tiles are
+//      written as zeros. No global atomics and no separate layer clearing.
+//   3. degrid mirrors it: the (T + K - 1)^2 subgrid of a (layer, tile) is
+//      gathered into LDS once, then one lane per primary record sums its K*K
+//      taps.
 //   Image-domain passes loop over all layers of a pass per pixel: the
 //   w-correction phase is computed once per pixel and layer, and the layer
 //   sum is Kahan-compensated in FP32.
@@ -74,6 +78,7 @@ struct GeomArgs {
   int tileSize, tilesX, tilesY;
   int rotate;        // apply the l/m shift phase rotation to samples
   double lShift, mShift;
+  int blocked;       // layer storage layout (cellIndex)
 };
 
 // ----------------------------------------------------------------- fill
@@ -149,84 +154,138 @@ __device__ __forceinline__ void waveCount(unsigned long long* counter,
   if ((threadIdx.x & 63) == 0 && mask) atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
 
-// Pass 1: place every sample, write its record and bin (unsorted) and
-// histogram the bins. Samples are visited channel-major (row fastest) so the
-// lanes of a wave hit different baselines and the histogram atomics spread.
+// Consecutive lanes of a wave whose samples fall in the same bin form runs
+// (samples are visited channel-fastest, so the channels of one row that stay
+// in one tile are neighbours). The first lane of a run carries its length,
+// so one atomic per run replaces one per sample: same-address atomics on the
+// dense uv centre otherwise serialise in L2.
+struct LaneRun {
+  int head;  // lane index of this lane's run head
+  int len;   // run length (valid on the head lane)
+};
+
+__device__ __forceinline__ LaneRun laneRun(uint32_t bin, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t prevBin = __shfl_up(bin, 1);
+  const unsigned long long validMask = __ballot(valid);
+  const bool prevValid = lane > 0 && ((validMask >> (lane - 1)) & 1ull);
+  const bool head = valid && (lane == 0 || !prevValid || prevBin != bin);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long ends = heads | ~validMask;
+  const unsigned long long after = (lane == 63) ? 0ull : ((ends >> (lane + 1)) << (lane + 1));
+  const int next = after ? (__ffsll((long long)after) - 1) : 64;
+  const unsigned long long upto = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+  LaneRun r;
+  r.len = next - lane;
+  r.head = upto ? (63 - __clzll((long long)upto)) : lane;
+  return r;
+}
+
+struct KindArgs {
+  LayerMap lm;
+  uint32_t* counts;              // [nbins + 1]
+  uint32_t* cursor;              // scatter: next free position per bin
+  uint2* recSorted;              // scatter: records ordered by bin
+  unsigned long long* counters;  // gridded, off grid, off layer
+};
+
+// The uv tiles a sample's footprint touches. Footprint cells run from
+// (xs, ys) (centre minus K/2, wrapped into the grid) for K cells per axis;
+// with tileSize >= K - 1 and tiles dividing the grid they touch at most two
+// tiles per axis: tile A holding the first cell and, if different, tile B
+// holding the last (possibly wrapped round to tile 0). For each tile the
+// footprint start relative to the tile origin is xo in [-(K-1), T).
+struct TileSpan {
+  int tA, tB;    // tB < 0: single tile
+  int xoA, xoB;
+};
+
+__device__ __forceinline__ TileSpan tileSpan(int xs, int K, int T, int W) {
+  TileSpan s;
+  s.tA = xs / T;
+  s.xoA = xs - s.tA * T;
+  const int last = xs + K - 1;
+  const int lastW = last >= W ? last - W : last;
+  const int tl = lastW / T;
+  if (tl == s.tA && last < W) {
+    s.tB = -1;
+    s.xoB = 0;
+  } else {
+    s.tB = tl;
+    s.xoB = (last < W) ? xs - tl * T : xs - W - tl * T;
+  }
+  return s;
+}
+
+// Bin and record of emission e (0: (A, A), 1: (B, A), 2: (A, B), 3: (B, B))
+// of a placed sample; returns false if that tile combination does not exist.
+__device__ __forceinline__ bool emission(const GeomArgs& ga, const Placement& p, int e,
+                                         uint32_t idx, uint32_t& bin, uint2& rec) {
+  int xs = p.x - ga.mid, ys = p.y - ga.mid;
+  if (xs < 0) xs += ga.W;
+  if (ys < 0) ys += ga.H;
+  const TileSpan sx = tileSpan(xs, ga.K, ga.tileSize, ga.W);
+  const TileSpan sy = tileSpan(ys, ga.K, ga.tileSize, ga.H);
+  const bool useBx = e & 1, useBy = e & 2;
+  if ((useBx && sx.tB < 0) || (useBy && sy.tB < 0)) return false;
+  const int tx = useBx ? sx.tB : sx.tA, xo = useBx ? sx.xoB : sx.xoA;
+  const int ty = useBy ? sy.tB : sy.tA, yo = useBy ? sy.xoB : sy.xoA;
+  bin = uint32_t((size_t(p.layer) * ga.tilesY + ty) * ga.tilesX + tx);
+  rec.x = idx | (uint32_t(e == 0) << 30) | (uint32_t(p.conj) << 31);
+  rec.y = uint32_t(xo + ga.K - 1) | (uint32_t(yo + ga.K - 1) << 6) |
+          (uint32_t(p.ky * ga.OS + p.kx) << 12);
+  return true;
+}
+
+__device__ __forceinline__ Placement placeSample(const GeomArgs& ga, const LayerMap& lm,
+                                                 const double* __restrict__ uvw,
+                                                 const double* __restrict__ wl, size_t i,
+                                                 int nchan) {
+  const size_t row = i / size_t(nchan);
+  const size_t ch = i - row * size_t(nchan);
+  const double lambda = wl[ch];
+  return place(ga, lm, uvw[row * 3] / lambda, uvw[row * 3 + 1] / lambda,
+               uvw[row * 3 + 2] / lambda);
+}
+
+// Pass 1: place every sample (FP64, AddDataSample arithmetic) and histogram
+// the (layer, tile) bins of all tiles its footprint touches. Sample
+// i = row * nchan + channel, the order of the visibility array; the lanes of a
+// run of consecutive samples in one bin add their count with one atomic.
+template <bool SCATTER>
 __global__ void __launch_bounds__(kThreads)
-binCountKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
-               size_t nrow, int nchan, GeomArgs ga, LayerMap lm,
-               uint2* __restrict__ recTmp, uint32_t* __restrict__ binTmp,
-               uint32_t* __restrict__ counts,
-               unsigned long long* __restrict__ counters) {
+binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
+          int nchan, GeomArgs ga, KindArgs k) {
   const size_t n = nrow * size_t(nchan);
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool gridded = false, offGrid = false, offLayer = false;
-  if (i < n) {
-    const size_t ch = i / nrow;
-    const size_t row = i - ch * nrow;
-    const double lambda = wl[ch];
-    const double u = uvw[row * 3] / lambda, v = uvw[row * 3 + 1] / lambda,
-                 w = uvw[row * 3 + 2] / lambda;
-    const Placement p = place(ga, lm, u, v, w);
-    uint32_t bin = 0xFFFFFFFFu;
-    if (p.layer < 0) {
-      offLayer = true;
-    } else if (!p.onGrid) {
-      offGrid = true;
-    } else {
-      int xs = p.x - ga.mid, ys = p.y - ga.mid;
-      if (xs < 0) xs += ga.W;
-      if (ys < 0) ys += ga.H;
-      const int tx = xs / ga.tileSize, ty = ys / ga.tileSize;
-      const int xo = xs - tx * ga.tileSize, yo = ys - ty * ga.tileSize;
-      bin = uint32_t((size_t(p.layer) * ga.tilesY + ty) * ga.tilesX + tx);
-      const uint32_t idx = uint32_t(row * nchan + ch);
-      uint2 rec;
-      rec.x = idx | (uint32_t(p.conj) << 31);
-      rec.y = uint32_t(xo) | (uint32_t(yo) << 6) |
-              (uint32_t(p.ky * ga.OS + p.kx) << 12);
-      recTmp[i] = rec;
-      atomicAdd(&counts[bin], 1u);
-      gridded = true;
-    }
-    binTmp[i] = bin;
+  const int lane = threadIdx.x & 63;
+  Placement p;
+  p.layer = -1;
+  p.onGrid = 0;
+  if (i < n) p = placeSample(ga, k.lm, uvw, wl, i, nchan);
+  const bool ok = i < n && p.layer >= 0 && p.onGrid;
+  if (!SCATTER) {
+    waveCount(&k.counters[0], ok);
+    waveCount(&k.counters[1], i < n && p.layer >= 0 && !p.onGrid);
+    waveCount(&k.counters[2], i < n && p.layer < 0);
   }
-  waveCount(&counters[0], gridded);
-  waveCount(&counters[1], offGrid);
-  waveCount(&counters[2], offLayer);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    uint32_t bin = 0xFFFFFFFFu;
+    uint2 rec = make_uint2(0u, 0u);
+    const bool has = ok && emission(ga, p, e, uint32_t(i), bin, rec);
+    const LaneRun r = laneRun(bin, has);
+    if (!SCATTER) {
+      if (has && r.head == lane) atomicAdd(&k.counts[bin], uint32_t(r.len));
+    } else {
+      uint32_t base = 0;
+      if (has && r.head == lane) base = atomicAdd(&k.cursor[bin], uint32_t(r.len));
+      base = __shfl(base, r.head);
+      if (has) k.recSorted[base + uint32_t(lane - r.head)] = rec;
+    }
+  }
 }
 
-// Pass 3: scatter records to their sorted position.
-__global__ void __launch_bounds__(kThreads)
-binScatterKernel(size_t n, const uint2* __restrict__ recTmp,
-                 const uint32_t* __restrict__ binTmp,
-                 uint32_t* __restrict__ cursor, uint2* __restrict__ recSorted) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t b = binTmp[i];
-  if (b == 0xFFFFFFFFu) return;
-  const uint32_t pos = atomicAdd(&cursor[b], 1u);
-  recSorted[pos] = recTmp[i];
-}
-
-__global__ void chunkCountKernel(const uint32_t* __restrict__ counts,
-                                 size_t nbins, uint32_t* __restrict__ nchunks) {
-  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b < nbins) nchunks[b] = (counts[b] + kChunkItems - 1) / kChunkItems;
-}
-
-__global__ void chunkEmitKernel(const uint32_t* __restrict__ counts,
-                                const uint32_t* __restrict__ binStart,
-                                const uint32_t* __restrict__ chunkStart,
-                                size_t nbins, uint4* __restrict__ chunks) {
-  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  const uint32_t c = counts[b];
-  uint32_t out = chunkStart[b];
-  for (uint32_t k = 0; k < c; k += kChunkItems, ++out)
-    chunks[out] = make_uint4(uint32_t(b), binStart[b] + k,
-                             min(kChunkItems, c - k), 0u);
-}
 
 // --------------------------------------------------------- exclusive scan
 constexpr int kScanItems = 8;
@@ -298,7 +357,7 @@ struct GridArgs {
   int layerLo, layerHi;  // layers of the pass (absolute)
   int nLayersPass;   // allocated layers per image block
   int tilesPerLayer;
-  int sg;            // subgrid edge = tileSize + K - 1
+  int sg;            // degrid subgrid edge = tileSize + K - 1
   size_t nchan;
 };
 
@@ -332,86 +391,188 @@ __device__ __forceinline__ void loadKernelTable(float* kt, const float* __restri
   }
 }
 
-template <int NIMG>
-__global__ void __launch_bounds__(kThreads)
-gridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
-                const uint32_t* __restrict__ chunkStart,
+__device__ __forceinline__ float readLaneF(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ int readLaneI(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+constexpr int kGridThreads = 512;
+
+// Gridding of one image kind, one (layer, uv tile) per workgroup at a time:
+// the workgroup owns the T x T tile, accumulates every record binned to it
+// in an LDS tile, and writes the tile to the layer once with plain stores
+// (ACCUM adds to what the layer holds, for callers that grid a pass in
+// several batches). No global atomics: tiles without samples are written
+// as zeros, which also clears the layer.
+// Each wave takes its records 64 at a time: lane l loads record l and its
+// visibilities (latency paid once per 64), then the wave walks them with the
+// values broadcast by readlane; lane t owns tap t of the K x K footprint (TG
+// groups of 64 taps for K > 8) and deposits only taps inside the tile.
+// While consecutive records keep the footprint in place (the channels of a
+// short baseline), each lane sums in registers and touches LDS only when the
+// footprint moves.
+template <int NIMG, int TG, bool ACCUM>
+__global__ void __launch_bounds__(kGridThreads)
+gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
                 const uint2* __restrict__ recs, const float2* __restrict__ vis,
                 const float* __restrict__ ktab, const __half2* __restrict__ khalf,
                 const double* __restrict__ uvw, const double* __restrict__ wl,
                 float2* __restrict__ layers) {
   extern __shared__ float smem[];
-  const int K = a.ga.K, OS = a.ga.OS, sg = a.sg, sg2 = a.sg * a.sg;
-  float* kt = smem;                  // [OS][K]
-  float* planes = smem + ((OS * K + 3) & ~3);  // [NIMG*2][sg2]
+  const int K = a.ga.K, OS = a.ga.OS, T = a.ga.tileSize, T2 = T * T;
+  float* kt = smem;                            // [OS][K]
+  float* tile = smem + ((OS * K + 3) & ~3);    // [NIMG*2][T*T]
   loadKernelTable(kt, ktab, khalf, OS * K);
-  const uint32_t lo = chunkStart[size_t(a.layerLo) * a.tilesPerLayer];
-  const uint32_t hi = chunkStart[size_t(a.layerHi) * a.tilesPerLayer];
+  const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
+  const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
   const int taps = K * K;
+  int tj[TG], ti[TG];
+  bool tapOk[TG];
+#pragma unroll
+  for (int q = 0; q < TG; ++q) {
+    const int t = lane + 64 * q;
+    tapOk[q] = t < taps;
+    tj[q] = tapOk[q] ? t / K : 0;
+    ti[q] = tapOk[q] ? t - tj[q] * K : 0;
+  }
   const size_t HW = size_t(a.ga.W) * a.ga.H;
-  for (uint32_t c = lo + blockIdx.x; c < hi; c += gridDim.x) {
-    const uint4 chunk = chunks[c];
-    const uint32_t bin = chunk.x, start = chunk.y, count = chunk.z;
+  __syncthreads();
+  for (uint32_t bin = binLo + blockIdx.x; bin < binHi; bin += gridDim.x) {
+    const uint32_t start = binStart[bin], count = binStart[bin + 1] - start;
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
-    for (int i = threadIdx.x; i < NIMG * 2 * sg2; i += blockDim.x) planes[i] = 0.0f;
+    if (ACCUM && count == 0) continue;  // block-uniform
+    for (int i = threadIdx.x; i < NIMG * 2 * T2; i += blockDim.x) tile[i] = 0.0f;
     __syncthreads();
-    for (uint32_t it = wave; it < count; it += nwaves) {
-      const uint2 rec = recs[start + it];
-      const uint32_t idx = rec.x & 0x7FFFFFFFu;
-      const bool conj = (rec.x >> 31) != 0;
-      const int xo = int(rec.y & 63u), yo = int((rec.y >> 6) & 63u);
-      const int kk = int(rec.y >> 12);
-      const int ky = kk / OS, kx = kk - ky * OS;
-      float2 val[NIMG];
+    for (uint32_t b0 = uint32_t(wave) * 64u; b0 < count; b0 += uint32_t(nwaves) * 64u) {
+      const uint32_t it = b0 + uint32_t(lane);
+      int key = 0, kxo = 0, kyo = 0;
+      float vx[NIMG], vy[NIMG];
 #pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi) {
-        float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
-        if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, -1.0);
-        if (conj) v.y = -v.y;
-        val[gi] = v;
-      }
-      for (int t = lane; t < taps; t += 64) {
-        const int j = t / K, i = t - j * K;
-        // yKernelValue * xKernel[i] (gridding/wstackinggridder.cpp:574)
-        const float kv = kt[ky * K + j] * kt[kx * K + i];
-        const int cell = (yo + j) * sg + xo + i;
+      for (int gi = 0; gi < NIMG; ++gi) vx[gi] = vy[gi] = 0.0f;
+      if (it < count) {
+        const uint2 rec = recs[start + it];
+        const uint32_t idx = rec.x & 0x3FFFFFFFu;
+        const bool conj = (rec.x >> 31) != 0;
+        key = int(rec.y & 4095u);  // (xo + K - 1) | (yo + K - 1) << 6
+        const int kk = int(rec.y >> 12);
+        const int ky = kk / OS;
+        kyo = ky * K;
+        kxo = (kk - ky * OS) * K;
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
-          atomicAdd(&planes[(2 * gi) * sg2 + cell], val[gi].x * kv);
-          atomicAdd(&planes[(2 * gi + 1) * sg2 + cell], val[gi].y * kv);
+          float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+          if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, -1.0);
+          vx[gi] = v.x;
+          vy[gi] = conj ? -v.y : v.y;
+        }
+      }
+      const int nb = min(64, int(count - b0));
+      float acc[TG][2 * NIMG];
+#pragma unroll
+      for (int q = 0; q < TG; ++q)
+#pragma unroll
+        for (int e = 0; e < 2 * NIMG; ++e) acc[q][e] = 0.0f;
+      int cur = readLaneI(key, 0);
+      for (int s = 0; s < nb; ++s) {
+        const int k = readLaneI(key, s);
+        if (k != cur) {
+          const int xo = (cur & 63) - (K - 1), yo = (cur >> 6) - (K - 1);
+#pragma unroll
+          for (int q = 0; q < TG; ++q) {
+            const int cx = xo + ti[q], cy = yo + tj[q];
+            if (tapOk[q] && cx >= 0 && cx < T && cy >= 0 && cy < T) {
+              const int cell = cy * T + cx;
+#pragma unroll
+              for (int gi = 0; gi < NIMG; ++gi) {
+                atomicAdd(&tile[(2 * gi) * T2 + cell], acc[q][2 * gi]);
+                atomicAdd(&tile[(2 * gi + 1) * T2 + cell], acc[q][2 * gi + 1]);
+              }
+            }
+#pragma unroll
+            for (int e = 0; e < 2 * NIMG; ++e) acc[q][e] = 0.0f;
+          }
+          cur = k;
+        }
+        const int kxs = readLaneI(kxo, s), kys = readLaneI(kyo, s);
+        float sx[NIMG], sy[NIMG];
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          sx[gi] = readLaneF(vx[gi], s);
+          sy[gi] = readLaneF(vy[gi], s);
+        }
+#pragma unroll
+        for (int q = 0; q < TG; ++q) {
+          // yKernelValue * xKernel[i] (gridding/wstackinggridder.cpp:574)
+          const float kv = kt[kys + tj[q]] * kt[kxs + ti[q]];
+#pragma unroll
+          for (int gi = 0; gi < NIMG; ++gi) {
+            acc[q][2 * gi] += sx[gi] * kv;
+            acc[q][2 * gi + 1] += sy[gi] * kv;
+          }
+        }
+      }
+      {
+        const int xo = (cur & 63) - (K - 1), yo = (cur >> 6) - (K - 1);
+#pragma unroll
+        for (int q = 0; q < TG; ++q) {
+          const int cx = xo + ti[q], cy = yo + tj[q];
+          if (tapOk[q] && cx >= 0 && cx < T && cy >= 0 && cy < T) {
+            const int cell = cy * T + cx;
+#pragma unroll
+            for (int gi = 0; gi < NIMG; ++gi) {
+              atomicAdd(&tile[(2 * gi) * T2 + cell], acc[q][2 * gi]);
+              atomicAdd(&tile[(2 * gi + 1) * T2 + cell], acc[q][2 * gi + 1]);
+            }
+          }
         }
       }
     }
     __syncthreads();
-    const int x0 = tx * a.ga.tileSize, y0 = ty * a.ga.tileSize;
-    for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x) {
-      const int cy = cell / sg, cx = cell - cy * sg;
-      const int gx = (x0 + cx) % a.ga.W, gy = (y0 + cy) % a.ga.H;
+    // write the tile in layer storage order (blocked: 4 rows x T columns
+    // contiguous per row block)
+    const int x0 = tx * T, y0 = ty * T;
+    for (int c = threadIdx.x; c < T2; c += blockDim.x) {
+      int cx, cy;
+      if (a.ga.blocked) {
+        const int rb = c / (4 * T), r4 = c - rb * 4 * T;
+        cx = r4 >> 2;
+        cy = rb * 4 + (r4 & 3);
+      } else {
+        cy = c / T;
+        cx = c - cy * T;
+      }
+      const size_t off = cellIndex(x0 + cx, y0 + cy, a.ga.W, a.ga.blocked);
+      const int cell = cy * T + cx;
 #pragma unroll
       for (int gi = 0; gi < NIMG; ++gi) {
-        const float re = planes[(2 * gi) * sg2 + cell];
-        const float im = planes[(2 * gi + 1) * sg2 + cell];
-        if (re != 0.0f || im != 0.0f) {
-          float2* dst = layers +
-                        (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW +
-                        size_t(gy) * a.ga.W + gx;
-          unsafeAtomicAdd(&dst->x, re);
-          unsafeAtomicAdd(&dst->y, im);
+        float2* dst = layers +
+                      (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
+        float2 v = make_float2(tile[(2 * gi) * T2 + cell], tile[(2 * gi + 1) * T2 + cell]);
+        if (ACCUM) {
+          const float2 o = *dst;
+          v.x += o.x;
+          v.y += o.y;
         }
+        *dst = v;
       }
     }
     __syncthreads();
   }
 }
 
+// Degridding of one image kind: per (layer, tile) the (T + K - 1)^2 subgrid
+// is gathered from the layer into LDS once; each lane then sums the K x K
+// taps of one primary record (the record of the tile holding the footprint
+// start) and writes its visibilities.
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
-degridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
-                  const uint32_t* __restrict__ chunkStart,
+degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
                   const uint2* __restrict__ recs, const float* __restrict__ ktab,
                   const __half2* __restrict__ khalf, const double* __restrict__ uvw,
                   const double* __restrict__ wl, const float2* __restrict__ layers,
@@ -421,17 +582,17 @@ degridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
   float* kt = smem;
   float2* sub = reinterpret_cast<float2*>(smem + ((OS * K + 3) & ~3));  // [NIMG][sg2]
   loadKernelTable(kt, ktab, khalf, OS * K);
-  const uint32_t lo = chunkStart[size_t(a.layerLo) * a.tilesPerLayer];
-  const uint32_t hi = chunkStart[size_t(a.layerHi) * a.tilesPerLayer];
+  const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
+  const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const size_t HW = size_t(a.ga.W) * a.ga.H;
-  for (uint32_t c = lo + blockIdx.x; c < hi; c += gridDim.x) {
-    const uint4 chunk = chunks[c];
-    const uint32_t bin = chunk.x, start = chunk.y, count = chunk.z;
+  for (uint32_t bin = binLo + blockIdx.x; bin < binHi; bin += gridDim.x) {
+    const uint32_t start = binStart[bin], count = binStart[bin + 1] - start;
+    if (count == 0) continue;  // block-uniform
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
     const int x0 = tx * a.ga.tileSize, y0 = ty * a.ga.tileSize;
-    __syncthreads();  // previous chunk's readers are done with sub
+    __syncthreads();  // previous bin's readers are done with sub
     for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x) {
       const int cy = cell / sg, cx = cell - cy * sg;
       const int gx = (x0 + cx) % a.ga.W, gy = (y0 + cy) % a.ga.H;
@@ -439,14 +600,15 @@ degridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
       for (int gi = 0; gi < NIMG; ++gi)
         sub[gi * sg2 + cell] =
             layers[(size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW +
-                   size_t(gy) * a.ga.W + gx];
+                   cellIndex(gx, gy, a.ga.W, a.ga.blocked)];
     }
     __syncthreads();
     for (uint32_t it = threadIdx.x; it < count; it += blockDim.x) {
       const uint2 rec = recs[start + it];
-      const uint32_t idx = rec.x & 0x7FFFFFFFu;
+      if (((rec.x >> 30) & 1u) == 0u) continue;  // secondary record of a split footprint
+      const uint32_t idx = rec.x & 0x3FFFFFFFu;
       const bool conj = (rec.x >> 31) != 0;
-      const int xo = int(rec.y & 63u), yo = int((rec.y >> 6) & 63u);
+      const int xo = int(rec.y & 63u) - (K - 1), yo = int((rec.y >> 6) & 63u) - (K - 1);
       const int kk = int(rec.y >> 12);
       const int ky = kk / OS, kx = kk - ky * OS;
       float2 acc[NIMG];
@@ -477,6 +639,7 @@ degridTilesKernel(GridArgs a, const uint4* __restrict__ chunks,
   }
 }
 
+
 // ----------------------------------------------------- plain per-channel
 struct PlainArgs {
   int W, H, K, OS, mid;
@@ -490,6 +653,7 @@ struct PlainArgs {
   float lShift, mShift;
   int ch;
   size_t nrow, nchan;
+  int blocked;
 };
 
 struct PlainPlacement {
@@ -571,7 +735,7 @@ gridPlainKernel(PlainArgs a, const double* __restrict__ uvw,
       if (cx < 0) cx += a.W;
       if (cx >= a.W) cx -= a.W;
       const float kv = yk * ktab[p.kx * a.K + i];
-      float2* dst = layer + size_t(cy) * a.W + cx;
+      float2* dst = layer + cellIndex(cx, cy, a.W, a.blocked);
       unsafeAtomicAdd(&dst->x, v.x * kv);
       unsafeAtomicAdd(&dst->y, v.y * kv);
     }
@@ -600,7 +764,7 @@ degridPlainKernel(PlainArgs a, const double* __restrict__ uvw,
       if (cx < 0) cx += a.W;
       if (cx >= a.W) cx -= a.W;
       const float kv = ktab[p.kx * a.K + i] * yk;
-      const float2 gv = layer[size_t(cy) * a.W + cx];
+      const float2 gv = layer[cellIndex(cx, cy, a.W, a.blocked)];
       acc.x += gv.x * kv;
       acc.y += gv.y * kv;
     }
@@ -618,6 +782,7 @@ struct ImgArgs {
   LayerMap maps[2];  // real, complex
   int isComplex[kMaxImages];
   int passStart, nLayersInPass, nLayersPass;
+  int blocked;
 };
 
 // n' = sqrt(1 - l^2 - m^2) - 1 inside the horizon, else 0
@@ -645,7 +810,8 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
   if (p >= HW) return;
   const int img = blockIdx.y;
   const int cplx = a.isComplex[img];
-  const int x = int(p % a.W), y = int(p / a.W);
+  int x, y;
+  cellOf(p, a.W, a.blocked, x, y);
   int ySrc = (a.H - y) + a.H / 2;
   if (ySrc >= a.H) ySrc -= a.H;
   int xSrc = x + a.W / 2;
@@ -694,13 +860,14 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
 
 // Plain form: one layer, FP32 phase from the float n' table.
 __global__ void __launch_bounds__(kThreads)
-projectPlainKernel(int W, int H, const float2* __restrict__ layer,
+projectPlainKernel(int W, int H, int blocked, const float2* __restrict__ layer,
                    const float* __restrict__ ntab, float w, int cplx,
                    float* __restrict__ sum, float* __restrict__ isum) {
   const size_t HW = size_t(W) * H;
   const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= HW) return;
-  const int x = int(p % W), y = int(p / W);
+  int x, y;
+  cellOf(p, W, blocked, x, y);
   int ySrc = (H - y) + H / 2;
   if (ySrc >= H) ySrc -= H;
   int xSrc = x + W / 2;
@@ -715,12 +882,13 @@ projectPlainKernel(int W, int H, const float2* __restrict__ layer,
 
 // n' tables as the reference stores them (T = float): order 0 for invert
 // (cpp:851-872), 1 for prediction (cpp:909-932).
-__global__ void nTableKernel(int W, int H, double dl, double dm, double ls,
-                             double ms, int order, float* __restrict__ out) {
+__global__ void nTableKernel(int W, int H, int blocked, double dl, double dm,
+                             double ls, double ms, int order, float* __restrict__ out) {
   const size_t HW = size_t(W) * H;
   const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= HW) return;
-  const int x = int(p % W), y = int(p / W);
+  int x, y;
+  cellOf(p, W, blocked, x, y);
   int xs, ys;
   if (order == 0) {
     ys = (H - y) + H / 2;
@@ -795,7 +963,8 @@ imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
   if (p >= HW) return;
   const int img = blockIdx.y;
   const int cplx = a.isComplex[img];
-  const int x = int(p % a.W), y = int(p / a.W);
+  int x, y;
+  cellOf(p, a.W, a.blocked, x, y);
   int yDest = y + a.H / 2;
   if (yDest >= a.H) yDest -= a.H;
   int xDest = (a.W - x) + a.W / 2;
@@ -835,14 +1004,15 @@ imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
 
 // Plain form: one layer of one image from the float predict-order n' table.
 __global__ void __launch_bounds__(kThreads)
-imageToLayerPlainKernel(int W, int H, const float* __restrict__ modelRe,
+imageToLayerPlainKernel(int W, int H, int blocked, const float* __restrict__ modelRe,
                         const float* __restrict__ modelIm,
                         const float* __restrict__ ntab, float w, int cplx,
                         float2* __restrict__ layer) {
   const size_t HW = size_t(W) * H;
   const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= HW) return;
-  const int x = int(p % W), y = int(p / W);
+  int x, y;
+  cellOf(p, W, blocked, x, y);
   int yDest = y + H / 2;
   if (yDest >= H) yDest -= H;
   int xDest = (W - x) + W / 2;
@@ -925,6 +1095,7 @@ GeomArgs geomArgs(const hws_gridder* g) {
               (g->cfg.l_shift != 0.0 || g->cfg.m_shift != 0.0);
   ga.lShift = g->cfg.l_shift;
   ga.mShift = g->cfg.m_shift;
+  ga.blocked = g->blocked;
   return ga;
 }
 
@@ -943,19 +1114,24 @@ ImgArgs imgArgs(const hws_gridder* g, int pass) {
   a.passStart = passLayerStart(g, pass);
   a.nLayersInPass = passLayerStart(g, pass + 1) - a.passStart;
   a.nLayersPass = g->maxLayersPerPass;
+  a.blocked = g->blocked;
   return a;
 }
 
 size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
-  const int sg = g->plan.tileSize + g->K - 1;
+  const int T = g->tileSize;
   const size_t kt = size_t((g->OS * g->K + 3) & ~3) * sizeof(float);
-  return kt + size_t(nimg) * sg * sg * (degrid ? sizeof(float2) : 2 * sizeof(float));
+  if (degrid) {
+    const int sg = T + g->K - 1;
+    return kt + size_t(nimg) * sg * sg * sizeof(float2);
+  }
+  return kt + size_t(nimg) * T * T * 2 * sizeof(float);
 }
 
-int persistentBlocks(const hws_gridder* g) {
+int persistentBlocks(const hws_gridder* g, int perCu) {
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->cfg.device);
-  return std::max(1, cus) * 8;
+  return std::max(1, cus) * perCu;
 }
 
 }  // namespace
@@ -973,31 +1149,25 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
   Plan& pl = g->plan;
   pl.valid = false;
   const size_t n = nrow * nchan;
-  if (n >= (size_t(1) << 31))
+  if (n >= (size_t(1) << 30))
     return setLastError(g, HWS_ERR_UNSUPPORTED,
-                        "a plan holds at most 2^31 - 1 (row, channel) samples");
+                        "a plan holds at most 2^30 - 1 (row, channel) samples");
   pl.nrow = nrow;
   pl.nchan = nchan;
   pl.uvw = dUvw;
   pl.wavelengths = dWl;
-  pl.tileSize = int(g->cfg.tile_size ? g->cfg.tile_size : 32);
-  pl.tilesX = (g->W + pl.tileSize - 1) / pl.tileSize;
-  pl.tilesY = (g->H + pl.tileSize - 1) / pl.tileSize;
+  pl.tileSize = g->tileSize;
+  pl.tilesX = g->W / pl.tileSize;
+  pl.tilesY = g->H / pl.tileSize;
   const size_t tpl = size_t(pl.tilesX) * pl.tilesY;
   pl.nbins = tpl * size_t(g->nW);
   if (pl.nbins >= 0xFFFFFFFFull)
     return setLastError(g, HWS_ERR_UNSUPPORTED, "too many (layer, tile) bins");
-
-  HWS_TRY(ensureAlloc(g, pl.recTmp, pl.tmpCapacity, n, "plan records"));
-  HWS_TRY(ensureAlloc(g, pl.binTmp, pl.binTmpCapacity, n, "plan bins"));
-  HWS_TRY(ensureAlloc(g, pl.counts, pl.countsCapacity, pl.nbins + 1, "plan counts"));
-  HWS_TRY(ensureAlloc(g, pl.scanTmp, pl.scanCapacity, pl.nbins + 1, "plan scan"));
   HWS_TRY(ensureAlloc(g, pl.scanScratch, pl.scanScratchCapacity, scanScratchSize(pl.nbins),
                       "plan scan scratch"));
-  if (!pl.counters) {
-    HWS_TRY(checkHip(g, hipMalloc(&pl.counters, 4 * sizeof(unsigned long long)), "plan counters"));
-  }
-  HWS_TRY(launchFill(g, pl.counters, 0u, 8));
+  if (!pl.counters)
+    HWS_TRY(checkHip(g, hipMalloc(&pl.counters, 8 * sizeof(unsigned long long)), "plan counters"));
+  HWS_TRY(launchFill(g, pl.counters, 0u, 16));
 
   const GeomArgs ga = geomArgs(g);
   for (int kind = 0; kind < 2; ++kind) {
@@ -1008,33 +1178,31 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
       if (int(g->isComplex[i]) == kind) kp.img[kp.nImg++] = i;
     kp.active = kp.nImg > 0;
     if (!kp.active) continue;
-    HWS_TRY(ensureAlloc(g, kp.recSorted, kp.capacity, n, "plan sorted records"));
+    HWS_TRY(ensureAlloc(g, kp.counts, kp.countsCapacity, pl.nbins + 1, "plan counts"));
     HWS_TRY(ensureAlloc(g, kp.binStart, kp.binCapacity, pl.nbins + 1, "plan bin offsets"));
-    HWS_TRY(ensureAlloc(g, kp.chunkStart, kp.chunkStartCapacity, pl.nbins + 1, "plan chunk offsets"));
-    HWS_TRY(ensureAlloc(g, kp.chunks, kp.chunkCapacity, n / kChunkItems + pl.nbins + 1, "plan chunks"));
-
-    HWS_TRY(launchFill(g, pl.counts, 0u, pl.nbins + 1));
-    const LayerMap lm = layerMap(g, kind);
+    HWS_TRY(launchFill(g, kp.counts, 0u, pl.nbins + 1));
+    KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind};
     if (n > 0) {
-      hipLaunchKernelGGL(binCountKernel, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
-                         dUvw, dWl, nrow, int(nchan), ga, lm, pl.recTmp, pl.binTmp,
-                         pl.counts, pl.counters);
+      hipLaunchKernelGGL(binKernel<false>, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
+                         dUvw, dWl, nrow, int(nchan), ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin count launch"));
     }
-    HWS_TRY(exclusiveScan(g, pl.counts, kp.binStart, pl.nbins, pl.scanScratch));
-    hipLaunchKernelGGL(chunkCountKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
-                       pl.counts, pl.nbins, pl.scanTmp);
-    HWS_TRY(checkHip(g, hipGetLastError(), "chunk count launch"));
-    HWS_TRY(exclusiveScan(g, pl.scanTmp, kp.chunkStart, pl.nbins, pl.scanScratch));
-    hipLaunchKernelGGL(chunkEmitKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
-                       pl.counts, kp.binStart, kp.chunkStart, pl.nbins, kp.chunks);
-    HWS_TRY(checkHip(g, hipGetLastError(), "chunk emit launch"));
+    HWS_TRY(exclusiveScan(g, kp.counts, kp.binStart, pl.nbins, pl.scanScratch));
+    // the record total is needed to size the sorted array
+    uint32_t total = 0;
+    HWS_TRY(checkHip(g, hipMemcpyAsync(&total, kp.binStart + pl.nbins, sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, g->stream), "record total"));
+    HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "record total"));
+    kp.nItems = total;
+    HWS_TRY(ensureAlloc(g, kp.recSorted, kp.capacity, size_t(total), "plan sorted records"));
     // cursor = bin offsets
-    HWS_TRY(checkHip(g, hipMemcpyAsync(pl.counts, kp.binStart, pl.nbins * sizeof(uint32_t),
+    HWS_TRY(checkHip(g, hipMemcpyAsync(kp.counts, kp.binStart, pl.nbins * sizeof(uint32_t),
                                        hipMemcpyDeviceToDevice, g->stream), "cursor copy"));
+    ka.cursor = kp.counts;
+    ka.recSorted = kp.recSorted;
     if (n > 0) {
-      hipLaunchKernelGGL(binScatterKernel, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
-                         n, pl.recTmp, pl.binTmp, pl.counts, kp.recSorted);
+      hipLaunchKernelGGL(binKernel<true>, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
+                         dUvw, dWl, nrow, int(nchan), ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin scatter launch"));
     }
   }
@@ -1042,9 +1210,9 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
   return HWS_OK;
 }
 
-hws_status gridBinned(hws_gridder* g, const float* dVis, int pass) {
-  Plan& pl = g->plan;
-  if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
+namespace {
+GridArgs gridArgs(const hws_gridder* g, int pass) {
+  const Plan& pl = g->plan;
   GridArgs a;
   a.ga = geomArgs(g);
   a.nImgTotal = g->nImg;
@@ -1055,8 +1223,17 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass) {
   a.tilesPerLayer = pl.tilesX * pl.tilesY;
   a.sg = pl.tileSize + g->K - 1;
   a.nchan = pl.nchan;
-  const int blocks = persistentBlocks(g);
+  return a;
+}
+}  // namespace
+
+hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate) {
+  Plan& pl = g->plan;
+  if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
+  GridArgs a = gridArgs(g, pass);
+  const int blocks = persistentBlocks(g, 4);
   const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
+  const int tg = (g->K * g->K + 63) / 64;
   for (int kind = 0; kind < 2; ++kind) {
     const KindPlan& kp = pl.kinds[kind];
     if (!kp.active) continue;
@@ -1064,19 +1241,36 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass) {
     for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = kp.img[i];
     const size_t lds = gridLdsBytes(g, kp.nImg, false);
     const float2* vis = reinterpret_cast<const float2*>(dVis);
-    switch (kp.nImg) {
-#define HWS_GRID_CASE(N)                                                               \
-  case N:                                                                              \
-    hipLaunchKernelGGL(gridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream, \
-                       a, kp.chunks, kp.chunkStart, kp.recSorted, vis, g->dKernel, khalf, \
-                       pl.uvw, pl.wavelengths, reinterpret_cast<float2*>(g->dLayers)); \
+#define HWS_GRID_LAUNCH(N, T, ACC)                                                           \
+  hipLaunchKernelGGL((gridTilesKernel<N, T, ACC>), dim3(blocks), dim3(kGridThreads), lds,    \
+                     g->stream, a, kp.binStart, kp.recSorted, vis, g->dKernel, khalf, pl.uvw, \
+                     pl.wavelengths, reinterpret_cast<float2*>(g->dLayers))
+#define HWS_GRID_TG(N, ACC)          \
+  if (tg == 1)                       \
+    HWS_GRID_LAUNCH(N, 1, ACC);      \
+  else if (tg == 2)                  \
+    HWS_GRID_LAUNCH(N, 2, ACC);      \
+  else if (tg == 3)                  \
+    HWS_GRID_LAUNCH(N, 3, ACC);      \
+  else                               \
+    HWS_GRID_LAUNCH(N, 4, ACC);
+#define HWS_GRID_CASE(N)             \
+  case N:                            \
+    if (accumulate) {                \
+      HWS_GRID_TG(N, true)           \
+    } else {                         \
+      HWS_GRID_TG(N, false)          \
+    }                                \
     break;
+    switch (kp.nImg) {
       HWS_GRID_CASE(1)
       HWS_GRID_CASE(2)
       HWS_GRID_CASE(3)
       HWS_GRID_CASE(4)
-#undef HWS_GRID_CASE
     }
+#undef HWS_GRID_CASE
+#undef HWS_GRID_TG
+#undef HWS_GRID_LAUNCH
     HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
   }
   return HWS_OK;
@@ -1085,17 +1279,8 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass) {
 hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
   Plan& pl = g->plan;
   if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
-  GridArgs a;
-  a.ga = geomArgs(g);
-  a.nImgTotal = g->nImg;
-  a.passStart = passLayerStart(g, pass);
-  a.layerLo = a.passStart;
-  a.layerHi = passLayerStart(g, pass + 1);
-  a.nLayersPass = g->maxLayersPerPass;
-  a.tilesPerLayer = pl.tilesX * pl.tilesY;
-  a.sg = pl.tileSize + g->K - 1;
-  a.nchan = pl.nchan;
-  const int blocks = persistentBlocks(g);
+  GridArgs a = gridArgs(g, pass);
+  const int blocks = persistentBlocks(g, 8);
   const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
   for (int kind = 0; kind < 2; ++kind) {
     const KindPlan& kp = pl.kinds[kind];
@@ -1105,11 +1290,11 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
     const size_t lds = gridLdsBytes(g, kp.nImg, true);
     float2* out = reinterpret_cast<float2*>(dVisOut);
     switch (kp.nImg) {
-#define HWS_DEGRID_CASE(N)                                                                \
-  case N:                                                                                 \
-    hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream, \
-                       a, kp.chunks, kp.chunkStart, kp.recSorted, g->dKernel, khalf, pl.uvw, \
-                       pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out); \
+#define HWS_DEGRID_CASE(N)                                                                 \
+  case N:                                                                                  \
+    hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream,  \
+                       a, kp.binStart, kp.recSorted, g->dKernel, khalf, pl.uvw,            \
+                       pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out);  \
     break;
       HWS_DEGRID_CASE(1)
       HWS_DEGRID_CASE(2)
@@ -1150,6 +1335,7 @@ PlainArgs plainArgs(const hws_gridder* g, int img, int pass, size_t nrow,
   a.ch = 0;
   a.nrow = nrow;
   a.nchan = nchan;
+  a.blocked = g->blocked;
   return a;
 }
 }  // namespace
@@ -1195,7 +1381,7 @@ hws_status ensureNTable(hws_gridder* g, int order) {
     HWS_TRY(checkHip(g, hipMalloc(&g->dNTable, HW * sizeof(float)), "n' table"));
   if (g->nTableKind == order) return HWS_OK;
   hipLaunchKernelGGL(nTableKernel, dim3(blocksFor(HW)), dim3(kThreads), 0, g->stream, g->W, g->H,
-                     g->cfg.pixel_size_x, g->cfg.pixel_size_y, g->cfg.l_shift, g->cfg.m_shift,
+                     int(g->blocked), g->cfg.pixel_size_x, g->cfg.pixel_size_y, g->cfg.l_shift, g->cfg.m_shift,
                      order, g->dNTable);
   HWS_TRY(checkHip(g, hipGetLastError(), "n' table launch"));
   g->nTableKind = order;
@@ -1215,7 +1401,7 @@ hws_status projectPass(hws_gridder* g, int pass) {
         const float2* layer = reinterpret_cast<const float2*>(g->dLayers) +
                               (size_t(img) * a.nLayersPass + li) * HW;
         hipLaunchKernelGGL(projectPlainKernel, dim3(blocksFor(HW)), dim3(kThreads), 0, g->stream,
-                           g->W, g->H, layer, g->dNTable, w, kind, g->dSum + img * HW,
+                           g->W, g->H, int(g->blocked), layer, g->dNTable, w, kind, g->dSum + img * HW,
                            g->dISum + img * HW);
         HWS_TRY(checkHip(g, hipGetLastError(), "plain project launch"));
       }
@@ -1277,7 +1463,7 @@ hws_status imageToLayers(hws_gridder* g, int pass) {
         const float w = float(layerToW(a.maps[kind], a.passStart + li));
         float2* layer = reinterpret_cast<float2*>(g->dLayers) + (size_t(img) * a.nLayersPass + li) * HW;
         hipLaunchKernelGGL(imageToLayerPlainKernel, dim3(blocksFor(HW)), dim3(kThreads), 0,
-                           g->stream, g->W, g->H, g->dModelRe + img * HW,
+                           g->stream, g->W, g->H, int(g->blocked), g->dModelRe + img * HW,
                            g->dModelIm ? g->dModelIm + img * HW : nullptr, g->dNTable, w, kind,
                            layer);
         HWS_TRY(checkHip(g, hipGetLastError(), "plain image-to-layer launch"));
