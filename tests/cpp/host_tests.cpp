@@ -1,0 +1,373 @@
+// Tests of the host-side plugin layer (wsclean_amd/host): the task flow of
+// GriddingTaskManager / ThreadedScheduler and HipWSMSGridder behind
+// MSGridderBase, checked against the oracle's restatement of WSMSGridder
+// (oracle/oracle.h) on the same in-memory observation.
+//   host_tests cpu   -- tests that need no GPU
+//   host_tests gpu   -- parity tests on the GPU
+#include "../../wsclean_amd/host/griddingtaskmanager.h"
+#include "../../wsclean_amd/host/hipwsmsgridder.h"
+#include "../../oracle/oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace wsclean_amd;
+
+namespace {
+int failures = 0;
+#define CHECK(cond)                                                                 \
+  do {                                                                              \
+    if (!(cond)) {                                                                  \
+      std::printf("  CHECK FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);         \
+      ++failures;                                                                   \
+    }                                                                               \
+  } while (0)
+#define CHECK_LT(a, b)                                                              \
+  do {                                                                              \
+    const double _a = (a), _b = (b);                                                \
+    if (!(_a < _b)) {                                                               \
+      std::printf("  CHECK FAILED %s:%d: %s = %g !< %g\n", __FILE__, __LINE__, #a,  \
+                  _a, _b);                                                          \
+      ++failures;                                                                   \
+    }                                                                               \
+  } while (0)
+
+void run(const char* name, const std::function<void()>& fn) {
+  const int before = failures;
+  try {
+    fn();
+  } catch (std::exception& e) {
+    std::printf("  exception: %s\n", e.what());
+    ++failures;
+  }
+  std::printf("%s %s\n", failures == before ? "PASS" : "FAIL", name);
+}
+
+double relL2(const float* a, const float* b, size_t n) {
+  double num = 0, den = 0;
+  for (size_t i = 0; i != n; ++i) {
+    num += (double(a[i]) - b[i]) * (double(a[i]) - b[i]);
+    den += double(b[i]) * b[i];
+  }
+  return std::sqrt(num / den);
+}
+
+// A seeded observation: nrow rows of uvw (m), nchan channels, two
+// correlations (XX real image, XY complex image), weights with some flags.
+struct Observation {
+  size_t nrow = 1500, nchan = 6;
+  std::vector<double> uvw, wl, times;
+  std::vector<std::complex<float>> xx, xy;
+  std::vector<float> weights;
+  double pixel = 0.003;
+  size_t width = 96;
+  Observation() {
+    std::mt19937_64 rng(4242);
+    std::uniform_real_distribution<double> uni(-1.0, 1.0);
+    std::normal_distribution<double> gauss(0.0, 1.0);
+    for (size_t ch = 0; ch != nchan; ++ch) wl.push_back(299792458.0 / (130e6 + 4e6 * ch));
+    const double umax = 0.40 / pixel * wl[nchan - 1];
+    for (size_t r = 0; r != nrow; ++r) {
+      uvw.push_back(uni(rng) * umax);
+      uvw.push_back(uni(rng) * umax);
+      uvw.push_back(uni(rng) * 40.0);
+      times.push_back(double(r / 50));
+      for (size_t ch = 0; ch != nchan; ++ch) {
+        xx.emplace_back(float(gauss(rng)), float(gauss(rng)));
+        xy.emplace_back(float(gauss(rng)), float(gauss(rng)));
+        weights.push_back((rng() % 17 == 0) ? 0.0f : float(0.5 + 0.5 * (uni(rng) + 1.0)));
+      }
+    }
+  }
+  MemoryMSProvider provider(bool complexPol) const {
+    return MemoryMSProvider(uvw, wl, complexPol ? xy : xx, weights, times);
+  }
+};
+
+Settings baseSettings(const Observation& o) {
+  Settings s;
+  s.gridderType = GridderType::HipWStacking;
+  s.paddedImageWidth = s.paddedImageHeight = o.width;
+  s.trimmedImageWidth = s.trimmedImageHeight = 80;
+  s.pixelScaleX = s.pixelScaleY = o.pixel;
+  s.antialiasingKernelSize = 7;
+  s.overSamplingFactor = 63;
+  s.weightMode = WeightMode::Uniform;
+  s.threadCount = 4;
+  s.memory = int64_t(8) << 30;
+  s.hipPrecision = HipPrecision::Reference;
+  return s;
+}
+
+orc_ms_settings oracleSettings(const Settings& s, bool complexPol, double ls = 0, double ms = 0) {
+  orc_ms_settings o{};
+  o.width = s.paddedImageWidth;
+  o.height = s.paddedImageHeight;
+  o.trim_width = s.trimmedImageWidth;
+  o.trim_height = s.trimmedImageHeight;
+  o.dl = s.pixelScaleX;
+  o.dm = s.pixelScaleY;
+  o.kernel_size = s.antialiasingKernelSize;
+  o.oversampling = s.overSamplingFactor;
+  o.grid_mode = int(s.gridMode);
+  o.nw_layers = s.nWLayers;
+  o.nw_factor = s.nWLayersFactor;
+  o.l_shift = ls;
+  o.m_shift = ms;
+  o.is_complex = complexPol;
+  o.ncpus = s.threadCount;
+  o.memory = double(s.memory);
+  return o;
+}
+
+void* oracleWeights(const Observation& o, const Settings& s) {
+  void* iw = orc_iw_create(int(s.weightMode), s.briggsRobustness, s.paddedImageWidth,
+                           s.paddedImageHeight, s.pixelScaleX, s.pixelScaleY, 0, 1.0);
+  orc_iw_grid(iw, o.nrow, o.nchan, 1, o.uvw.data(), o.wl.data(), o.weights.data());
+  orc_iw_finish(iw);
+  return iw;
+}
+
+std::shared_ptr<ImageWeights> hostWeights(const Observation& o, const Settings& s,
+                                          MSProvider& p) {
+  auto w = std::make_shared<ImageWeights>(s.weightMode, s.briggsRobustness, s.paddedImageWidth,
+                                          s.paddedImageHeight, s.pixelScaleX, s.pixelScaleY);
+  w->Grid(p);
+  w->FinishGridding();
+  return w;
+}
+
+GriddingResult invertTask(const Settings& s, MSProvider& p, std::shared_ptr<ImageWeights> w,
+                          bool complexPol, double ls = 0, double ms = 0, bool psf = false) {
+  std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
+  GriddingTask t;
+  t.operation = GriddingTask::Invert;
+  t.complexPolarization = complexPol;
+  t.imageWeights = w;
+  t.msList = {&p};
+  t.l_shift = ls;
+  t.m_shift = ms;
+  t.imagePSF = psf;
+  return tm->RunDirect(std::move(t));
+}
+
+void cpuTests() {
+  Observation o;
+  run("image_weights_match_oracle", [&] {
+    for (WeightMode m : {WeightMode::Natural, WeightMode::Uniform, WeightMode::Briggs}) {
+      Settings s = baseSettings(o);
+      s.weightMode = m;
+      s.briggsRobustness = 0.5;
+      MemoryMSProvider p = o.provider(false);
+      auto w = hostWeights(o, s, p);
+      void* iw = oracleWeights(o, s);
+      std::vector<double> ref(orc_iw_width(iw) * orc_iw_height(iw) / 2);
+      orc_iw_grid_values(iw, ref.data());
+      CHECK(ref.size() == w->GridValues().size());
+      bool same = true;
+      for (size_t i = 0; i != ref.size(); ++i) same = same && ref[i] == w->GridValues()[i];
+      CHECK(same);
+      orc_iw_destroy(iw);
+    }
+  });
+  run("unavailable_gridder_type_throws", [&] {
+    Settings s = baseSettings(o);
+    s.gridderType = GridderType::WGridder;
+    MemoryMSProvider p = o.provider(false);
+    bool thrown = false;
+    try {
+      invertTask(s, p, hostWeights(o, s, p), false);
+    } catch (std::runtime_error& e) {
+      thrown = std::string(e.what()).find("hip-wstacking") != std::string::npos;
+    }
+    CHECK(thrown);
+  });
+  run("threaded_scheduler_rethrows_task_exception_at_finish", [&] {
+    Settings s = baseSettings(o);
+    s.gridderType = GridderType::IDG;
+    s.parallelGridding = 2;
+    MemoryMSProvider p = o.provider(false);
+    auto w = hostWeights(o, s, p);
+    std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
+    int callbacks = 0;
+    for (int i = 0; i != 2; ++i) {
+      GriddingTask t;
+      t.imageWeights = w;
+      t.msList = {&p};
+      try {
+        tm->Run(std::move(t), [&](GriddingResult&) { ++callbacks; });
+      } catch (std::runtime_error&) {
+      }
+    }
+    bool thrown = false;
+    try {
+      tm->Finish();
+    } catch (std::runtime_error&) {
+      thrown = true;
+    }
+    CHECK(thrown || callbacks == 0);
+    CHECK(callbacks == 0);
+  });
+  run("predict_checks_image_count", [&] {
+    Settings s = baseSettings(o);
+    HipWSMSGridder g(s, Resources(1, 1 << 30));
+    g.SetIsComplex(true);
+    bool thrown = false;
+    try {
+      std::vector<Image> one(1, Image(96, 96));
+      g.Predict(std::move(one));
+    } catch (std::runtime_error& e) {
+      thrown = std::string(e.what()) == "Missing imaginary in complex prediction";
+    }
+    CHECK(thrown);
+  });
+}
+
+void gpuTests() {
+  Observation o;
+  run("invert_real_matches_oracle_wsmsgridder", [&] {
+    Settings s = baseSettings(o);
+    MemoryMSProvider p = o.provider(false);
+    GriddingResult r = invertTask(s, p, hostWeights(o, s, p), false);
+    orc_ms_settings os = oracleSettings(s, false);
+    void* iw = oracleWeights(o, s);
+    std::vector<float> ref(80 * 80);
+    orc_invert_stats st{};
+    CHECK(orc_ms_invert(0, &os, o.nrow, o.nchan, o.uvw.data(), o.wl.data(),
+                        reinterpret_cast<const float*>(o.xx.data()), o.weights.data(), iw,
+                        ref.data(), nullptr, &st) == 0);
+    orc_iw_destroy(iw);
+    CHECK(r.images.size() == 1 && r.images[0].Size() == 80 * 80);
+    const double e = relL2(r.images[0].Data(), ref.data(), ref.size());
+    std::printf("  image rel L2 vs oracle = %.3g\n", e);
+    CHECK_LT(e, 1e-5);
+    CHECK(r.griddedVisibilityCount == st.gridded_count);
+    CHECK(r.actualWGridSize == st.nw);
+    CHECK_LT(std::fabs(r.imageWeight - st.total_weight), 1e-9 * st.total_weight);
+    CHECK_LT(std::fabs(r.visibilityWeightSum - st.visibility_weight_sum),
+             1e-9 * st.visibility_weight_sum);
+  });
+  run("invert_complex_xy_matches_oracle", [&] {
+    Settings s = baseSettings(o);
+    s.weightMode = WeightMode::Briggs;
+    MemoryMSProvider p = o.provider(true);
+    GriddingResult r = invertTask(s, p, hostWeights(o, s, p), true);
+    orc_ms_settings os = oracleSettings(s, true);
+    void* iw = oracleWeights(o, s);
+    std::vector<float> re(80 * 80), im(80 * 80);
+    orc_invert_stats st{};
+    CHECK(orc_ms_invert(0, &os, o.nrow, o.nchan, o.uvw.data(), o.wl.data(),
+                        reinterpret_cast<const float*>(o.xy.data()), o.weights.data(), iw,
+                        re.data(), im.data(), &st) == 0);
+    orc_iw_destroy(iw);
+    CHECK(r.images.size() == 2);
+    CHECK_LT(relL2(r.images[0].Data(), re.data(), re.size()), 1e-5);
+    CHECK_LT(relL2(r.images[1].Data(), im.data(), im.size()), 1e-5);
+    CHECK(r.actualWGridSize == st.nw);
+  });
+  run("predict_matches_oracle_wsmsgridder", [&] {
+    Settings s = baseSettings(o);
+    MemoryMSProvider p = o.provider(false);
+    auto w = hostWeights(o, s, p);
+    std::mt19937 rng(7);
+    std::normal_distribution<float> gauss;
+    Image model(80, 80);
+    for (size_t i = 0; i != model.Size(); ++i) model[i] = gauss(rng);
+    std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
+    GriddingTask t;
+    t.operation = GriddingTask::Predict;
+    t.imageWeights = w;
+    t.msList = {&p};
+    t.modelImages.push_back(model);
+    tm->RunDirect(std::move(t));
+    orc_ms_settings os = oracleSettings(s, false);
+    void* iw = oracleWeights(o, s);
+    std::vector<std::complex<float>> ref(o.nrow * o.nchan, std::complex<float>(0, 0));
+    orc_invert_stats st{};
+    CHECK(orc_ms_predict(0, &os, o.nrow, o.nchan, o.uvw.data(), o.wl.data(), o.weights.data(),
+                         iw, model.Data(), nullptr, reinterpret_cast<float*>(ref.data()),
+                         &st) == 0);
+    orc_iw_destroy(iw);
+    double num = 0, den = 0;
+    for (size_t i = 0; i != ref.size(); ++i) {
+      num += std::norm(std::complex<double>(p.Model()[i]) - std::complex<double>(ref[i]));
+      den += std::norm(std::complex<double>(ref[i]));
+    }
+    std::printf("  model rel L2 vs oracle = %.3g\n", std::sqrt(num / den));
+    CHECK_LT(std::sqrt(num / den), 1e-5);
+  });
+  run("shifted_phase_centre_matches_oracle", [&] {
+    Settings s = baseSettings(o);
+    MemoryMSProvider p = o.provider(false);
+    const double ls = 0.011, ms = -0.007;
+    GriddingResult r = invertTask(s, p, hostWeights(o, s, p), false, ls, ms);
+    orc_ms_settings os = oracleSettings(s, false, ls, ms);
+    void* iw = oracleWeights(o, s);
+    std::vector<float> ref(80 * 80);
+    CHECK(orc_ms_invert(0, &os, o.nrow, o.nchan, o.uvw.data(), o.wl.data(),
+                        reinterpret_cast<const float*>(o.xx.data()), o.weights.data(), iw,
+                        ref.data(), nullptr, nullptr) == 0);
+    orc_iw_destroy(iw);
+    CHECK_LT(relL2(r.images[0].Data(), ref.data(), ref.size()), 1e-5);
+  });
+  run("threaded_scheduler_matches_serial", [&] {
+    Settings s = baseSettings(o);
+    MemoryMSProvider pxx = o.provider(false), pxy = o.provider(true);
+    auto w = hostWeights(o, s, pxx);
+    GriddingResult serialXX = invertTask(s, pxx, w, false);
+    GriddingResult serialXY = invertTask(s, pxy, w, true);
+    Settings sp = s;
+    sp.parallelGridding = 2;
+    std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(sp);
+    std::vector<std::vector<Image>> got(2);
+    for (int i = 0; i != 2; ++i) {
+      GriddingTask t;
+      t.imageWeights = w;
+      t.complexPolarization = i == 1;
+      t.msList = {i == 1 ? &pxy : &pxx};
+      tm->Run(std::move(t), [&, i](GriddingResult& r) { got[i] = std::move(r.images); });
+    }
+    tm->Finish();
+    CHECK(got[0].size() == 1 && got[1].size() == 2);
+    CHECK_LT(relL2(got[0][0].Data(), serialXX.images[0].Data(), 80 * 80), 1e-6);
+    CHECK_LT(relL2(got[1][1].Data(), serialXY.images[1].Data(), 80 * 80), 1e-6);
+  });
+  run("multipass_budget_gives_same_image", [&] {
+    Settings s = baseSettings(o);
+    s.nWLayers = 16;
+    MemoryMSProvider p = o.provider(false);
+    auto w = hostWeights(o, s, p);
+    GriddingResult one = invertTask(s, p, w, false);
+    s.hipMaxLayerMemory = 96.0 * 96 * 8 * 5;  // five layers per pass
+    GriddingResult many = invertTask(s, p, w, false);
+    CHECK_LT(relL2(many.images[0].Data(), one.images[0].Data(), 80 * 80), 1e-6);
+  });
+  run("psf_peak_is_one", [&] {
+    Settings s = baseSettings(o);
+    s.weightMode = WeightMode::Natural;
+    MemoryMSProvider p = o.provider(false);
+    GriddingResult r = invertTask(s, p, hostWeights(o, s, p), false, 0, 0, true);
+    const float peak = r.images[0][40 + 40 * 80];
+    float maxv = -1e30f;
+    for (size_t i = 0; i != r.images[0].Size(); ++i) maxv = std::max(maxv, r.images[0][i]);
+    std::printf("  psf centre = %.6f, max = %.6f\n", peak, maxv);
+    CHECK_LT(std::fabs(peak - 1.0), 2e-3);
+    CHECK(maxv == peak);
+  });
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  if (mode == "cpu" || mode == "all") cpuTests();
+  if (mode == "gpu" || mode == "all") gpuTests();
+  std::printf("%d failure(s)\n", failures);
+  return failures == 0 ? 0 : 1;
+}

@@ -1,0 +1,36 @@
+"""Host plugin layer (wsclean_amd/host: GriddingTaskManager,
+ThreadedScheduler, MSGridderBase, HipWSMSGridder, ImageWeights) through its
+C++ test driver, tests/cpp/host_tests.cpp, which checks it against the
+oracle's restatement of WSMSGridder (gridding/wsmsgridder.cpp:1-499)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "host_tests")
+
+
+def _driver():
+    if not os.path.exists(DRIVER):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")], check=True)
+    return DRIVER
+
+
+def _run(mode):
+    r = subprocess.run([_driver(), mode], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+    return r.stdout
+
+
+def test_host_layer_cpu():
+    out = _run("cpu")
+    assert out.count("PASS") == 4
+
+
+@pytest.mark.gpu
+def test_host_layer_gpu():
+    out = _run("gpu")
+    assert out.count("PASS") == 7

@@ -1,4 +1,5 @@
-"""In-tree build of the HIP library (libhipwstacking.so, gfx950)."""
+"""In-tree builds: the HIP library (libhipwstacking.so, gfx950) and the
+host plugin layer over its C ABI (libwsclean_amd_host.so)."""
 from __future__ import annotations
 
 import os
@@ -7,11 +8,21 @@ import subprocess
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def build(arch: str = "gfx950", force: bool = True) -> str:
-    """Compile wsclean_amd/csrc into wsclean_amd/libhipwstacking.so."""
-    csrc = os.path.join(_HERE, "csrc")
-    args = ["make", "-s", "-C", csrc, f"ARCH={arch}"]
+def _make(directory: str, force: bool, *extra: str) -> None:
+    args = ["make", "-s", "-C", directory, *extra]
     if force:
         args.insert(1, "-B")
     subprocess.run(args, check=True)
+
+
+def build(arch: str = "gfx950", force: bool = True) -> str:
+    """Compile wsclean_amd/csrc into wsclean_amd/libhipwstacking.so."""
+    _make(os.path.join(_HERE, "csrc"), force, f"ARCH={arch}")
     return os.path.join(_HERE, "libhipwstacking.so")
+
+
+def build_host(force: bool = True) -> str:
+    """Compile wsclean_amd/host into wsclean_amd/libwsclean_amd_host.so
+    (needs libhipwstacking.so)."""
+    _make(os.path.join(_HERE, "host"), force)
+    return os.path.join(_HERE, "libwsclean_amd_host.so")
