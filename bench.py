@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--tile", type=int, default=0)
     p.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-accuracy", action="store_true",
+                   help="skip the error report against the FP64 direct transform")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -203,10 +205,13 @@ def main():
                    ("image", ph["project"] + ph["img2layer"]), ("bin", ph["bin"]),
                    key=lambda x: x[1])[0]
     roofline = {
-        "bound": "hbm", "kernel": "layer 2-D FFT (fftRowsPow2Kernel + fftColsPow2Kernel)",
+        "bound": "hbm",
+        "kernel": "layer 2-D FFT, blocked layout (fftRowsBlockedKernel + fftColsBlockedKernel)",
         "achieved": round(fft_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(fft_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-        "algorithmic_bytes_per_launch_pair": bytes_per_fft * Workload.NW,
+        "algorithmic_bytes_per_layer_transform": bytes_per_fft,
+        "layer_transforms_per_direction": n_fft_2d,
+        "avg_ms_per_layer_transform": round(fft_ms / n_fft_2d, 5) if n_fft_2d else None,
     }
 
     plain = None
@@ -218,6 +223,16 @@ def main():
             log(f"plain: {ms_plain * 1e3:.1f} ms/step")
         except Exception as e:  # reported, never hidden
             plain = {"error": str(e)}
+
+    accuracy = None
+    if not args.no_accuracy:
+        from wsclean_amd import accuracy as acc
+        t = time.perf_counter()
+        try:
+            accuracy = acc.report(device="cuda:0")
+        except Exception as e:  # reported, never hidden
+            accuracy = {"error": str(e)}
+        log(f"accuracy report in {time.perf_counter() - t:.1f} s: {accuracy}")
 
     cpu = None
     if not args.no_cpu:
@@ -247,6 +262,9 @@ def main():
         "samples_off_grid": st_inv["samples_off_grid"] if st_inv else None,
         "plain_form": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain and "value" in plain else None,
+        "rms_err_vs_fp64_dft": (accuracy.get(f"{args.precision}_os63", {}).get("image_rms_rel")
+                                if accuracy else None),
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

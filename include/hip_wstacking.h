@@ -216,6 +216,41 @@ hws_status hws_get_stats(const hws_gridder* g, hws_stats* out);
 /* Wait for all work of the context. */
 hws_status hws_synchronize(hws_gridder* g);
 
+/* ---- FP64 direct Fourier transform (no context) ----
+ * Pixel (x, y) of the width x height image has l = (W/2 - x) dl + l_shift,
+ * m = (y - H/2) dm + m_shift, n' = sqrt(1 - l^2 - m^2) - 1 (0 outside the
+ * horizon); phi = u l + v m + w n' with uvw in wavelengths (uvw_m / lambda).
+ * Samples are the (row, channel) pairs; sample (r, c) of d_vis is the
+ * complex64 at index (r * nchan + c) * vis_stride (vis_stride = n_images
+ * and an offset pointer select one image of a [row][chan][image] array).
+ * Work is enqueued on hip_stream (NULL: the null stream) of `device`.
+ *
+ * Replaces: DirectMSGridder<double>::Invert / gridSample /
+ * initializeSqrtLMLookupTable (gridding/directmsgridder.cpp:16-74, 76-114,
+ * 175-196): d_image[y * W + x] = mult * sum Re(V exp(-2 pi i phi)), FP64.
+ * Returns once the transform is enqueued. */
+hws_status hws_direct_invert_device(int device, void* hip_stream, size_t width,
+                                    size_t height, double dl, double dm,
+                                    double l_shift, double m_shift, size_t nrow,
+                                    size_t nchan, const double* d_uvw_m,
+                                    const double* d_wavelengths,
+                                    const float* d_vis, size_t vis_stride,
+                                    double multiplication_factor,
+                                    double* d_image);
+/* The adjoint of hws_direct_invert_device (with mult = 1), which the
+ * reference leaves unimplemented (DirectMSGridder::Predict throws,
+ * gridding/directmsgridder.cpp:167-171): sample = sum over the non-zero
+ * pixels of I(x, y) exp(+2 pi i phi), summed in FP64, stored as complex64.
+ * Allocates scratch for the non-zero pixel list and returns after the
+ * transform has completed. */
+hws_status hws_direct_predict_device(int device, void* hip_stream, size_t width,
+                                     size_t height, double dl, double dm,
+                                     double l_shift, double m_shift,
+                                     const float* d_image, size_t nrow,
+                                     size_t nchan, const double* d_uvw_m,
+                                     const double* d_wavelengths,
+                                     float* d_vis_out, size_t vis_stride);
+
 #ifdef __cplusplus
 }
 #endif

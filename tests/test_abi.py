@@ -69,3 +69,17 @@ def test_null_context_calls_fail_cleanly():
     assert L.hws_start_inversion_pass(None, 0) == 1
     assert L.hws_finish_inversion_pass(None) == 1
     assert L.hws_synchronize(None) == 1
+
+
+def test_direct_transform_rejects_bad_arguments_before_device_use():
+    from wsclean_amd import _lib
+    L = _lib.load()
+    st = L.hws_direct_invert_device(0, None, 0, 16, 0.01, 0.01, 0.0, 0.0, 0, 1, None, None,
+                                    None, 1, 1.0, None)
+    assert st == 1 and b"image size" in L.hws_last_error(None)
+    st = L.hws_direct_predict_device(0, None, 16, 16, -0.01, 0.01, 0.0, 0.0, None, 0, 1, None,
+                                     None, None, 1)
+    assert st == 1 and b"pixel sizes" in L.hws_last_error(None)
+    st = L.hws_direct_invert_device(0, None, 16, 16, 0.01, 0.01, 0.0, 0.0, 0, 1, None, None,
+                                    None, 1, 1.0, None)
+    assert st == 1 and b"null" in L.hws_last_error(None)

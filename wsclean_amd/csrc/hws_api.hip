@@ -11,9 +11,14 @@
 
 using namespace hws;
 
-namespace {
+namespace hws {
+std::string& threadError() {
+  thread_local std::string msg;
+  return msg;
+}
+}  // namespace hws
 
-thread_local std::string gGlobalError;
+namespace {
 
 #define HWS_TRY(expr)            \
   do {                           \
@@ -137,7 +142,7 @@ hws_status fail(hws_gridder* g, hws_status s, const std::string& msg) {
   if (g)
     g->err = msg;
   else
-    gGlobalError = msg;
+    threadError() = msg;
   return s;
 }
 
@@ -232,7 +237,7 @@ const char* hws_status_string(hws_status s) {
 }
 
 const char* hws_last_error(const hws_gridder* g) {
-  return g ? g->err.c_str() : gGlobalError.c_str();
+  return g ? g->err.c_str() : threadError().c_str();
 }
 
 hws_status hws_get_kernel(int kernel_mode, size_t oversampling, size_t size,
@@ -295,7 +300,7 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   auto bail = [&](hws_status s) {
     const std::string msg = g->err;
     hws_destroy(g);
-    gGlobalError = msg;
+    threadError() = msg;
     return s;
   };
   if (hipSetDevice(c->device) != hipSuccess) return bail(fail(g, HWS_ERR_HIP, "hipSetDevice failed"));
@@ -349,12 +354,12 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
       if (g->W % t == 0 && g->H % t == 0 && t >= K - 1 && std::abs(t - want) <= std::abs(best - want))
         best = t;
     if (c->tile_size && best != want) {
-      gGlobalError = "tile_size must divide width and height, be a multiple of 4 and >= K - 1";
+      threadError() = "tile_size must divide width and height, be a multiple of 4 and >= K - 1";
       hws_destroy(g);
       return HWS_ERR_INVALID_ARGUMENT;
     }
     if (best == 0 && c->precision != HWS_PRECISION_PLAIN_FP32) {
-      gGlobalError = "no uv tile size divides both image dimensions";
+      threadError() = "no uv tile size divides both image dimensions";
       hws_destroy(g);
       return HWS_ERR_UNSUPPORTED;
     }

@@ -148,6 +148,9 @@ struct hws_gridder {
 namespace hws {
 
 // kernels / launchers (hws_kernels.hip)
+// Message of the last failure of a call made without a context (per thread).
+std::string& threadError();
+// Records msg as g's last error (the thread's when g is null); returns s.
 hws_status setLastError(hws_gridder* g, hws_status s, const std::string& msg);
 hws_status checkHip(hws_gridder* g, hipError_t e, const char* what);
 

@@ -36,7 +36,10 @@
 namespace hws {
 
 hws_status setLastError(hws_gridder* g, hws_status s, const std::string& msg) {
-  if (g) g->err = msg;
+  if (g)
+    g->err = msg;
+  else
+    threadError() = msg;
   return s;
 }
 
