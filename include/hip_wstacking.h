@@ -98,6 +98,10 @@ typedef struct hws_config {
    * of WStackingGridder::SetDenormalPhaseCentre. */
   int rotate_visibilities;
   size_t tile_size;             /* uv tile edge for binning; 0 = default */
+  /* Records per work item of the binned grid/degrid kernels: a (layer, tile)
+   * bin with more records is split into several items (whose tiles are
+   * summed in a fixed order afterwards). 0 = chosen from the sample count. */
+  size_t work_chunk;
 } hws_config;
 
 typedef struct hws_gridder hws_gridder;
@@ -185,8 +189,9 @@ hws_status hws_get_kernel(int kernel_mode, size_t oversampling, size_t size,
 /* ---- device-resident batched forms (no reference counterpart as single
  * calls; they run the same protocol as above over every pass) ----
  * All pointers are device pointers. Work is ordered on the stream set with
- * hws_set_stream (default: a stream the context owns); calls return once the
- * work is enqueued, except where a host-visible result is needed.
+ * hws_set_stream (default: a non-blocking stream the context owns; NULL
+ * selects HIP's null stream); calls return once the work is enqueued, except
+ * where a host-visible result is needed.
  * The context must have been prepared with hws_prepare_w_layers. */
 hws_status hws_set_stream(hws_gridder* g, void* hip_stream);
 /* Bin (row, channel) samples by (w-layer, uv tile) for both image kinds; the

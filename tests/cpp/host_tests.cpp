@@ -319,6 +319,9 @@ void gpuTests() {
   });
   run("threaded_scheduler_matches_serial", [&] {
     Settings s = baseSettings(o);
+    // the suggested w-grid size depends on the task's CPU share
+    // (wsmsgridder.cpp:97-123), which the scheduler halves: fix it
+    s.nWLayers = 8;
     MemoryMSProvider pxx = o.provider(false), pxy = o.provider(true);
     auto w = hostWeights(o, s, pxx);
     GriddingResult serialXX = invertTask(s, pxx, w, false);

@@ -83,3 +83,6 @@ def test_direct_transform_rejects_bad_arguments_before_device_use():
     st = L.hws_direct_invert_device(0, None, 16, 16, 0.01, 0.01, 0.0, 0.0, 0, 1, None, None,
                                     None, 1, 1.0, None)
     assert st == 1 and b"null" in L.hws_last_error(None)
+    st = L.hws_direct_invert_device(0, None, 33, 16, 0.01, 0.01, 0.0, 0.0, 0, 1, None, None,
+                                    None, 1, 1.0, None)
+    assert st == 1 and b"even" in L.hws_last_error(None)

@@ -327,3 +327,53 @@ def test_blocked_layer_layout_sizes(hws, oracle, shape, precision):
     ok = ~np.isnan(v_o)
     assert np.array_equal(np.isnan(v), ~ok)
     assert rel(v[ok], v_o[ok]) < TOL_IMG[precision]
+
+
+@pytest.mark.parametrize("precision", ["reference", "mixed"])
+def test_split_bins_match_oracle(hws, oracle, precision):
+    # a dense uv centre: every sample falls in a few (layer, tile) bins, and a
+    # work chunk of 37 records splits each of them into many work items whose
+    # tiles are summed afterwards (grid) or which gather separately (degrid)
+    import torch
+    w = 128
+    uvw, wl, vis, dl = case(12, w, nrow=1500, nchan=4, frac=0.06, nimg=2)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    nw = 4
+    img_o, lay_o = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis[:, :, 0].reshape(-1), nw, 0.0,
+                                 maxw)
+    # host protocol, two add_data batches: the accumulate path
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision=precision, n_images=2,
+                             work_chunk=37)
+    g.prepare_w_layers(nw, 0.0, 0.0, maxw)
+    g.start_inversion_pass(0)
+    g.add_data(uvw[:700], wl, vis[:700])
+    g.add_data(uvw[700:], wl, vis[700:])
+    lay = [g.get_gridded_uv_layer(i, image=0) for i in range(nw)]
+    g.finish_inversion_pass()
+    re, _ = g.finalize_image(1.0)
+    for i in range(nw):
+        if np.abs(lay_o[i]).sum() > 0:
+            assert rel(lay[i], lay_o[i]) < TOL_LAYER[precision], i
+    assert rel(re[0], img_o) < TOL_IMG[precision]
+    # device form: the tile-writing path; both images, then predict
+    dev = torch.device("cuda:0")
+    d = hws.DeviceGridder(w, w, dl, dl, 7, 63, precision=precision, n_images=2, work_chunk=37)
+    d.prepare_w_layers(nw, 0.0, maxw)
+    d_uvw, d_wl = torch.from_numpy(uvw).to(dev), torch.from_numpy(wl).to(dev)
+    d_vis = torch.from_numpy(vis).to(dev)
+    out = torch.zeros((2, w, w), dtype=torch.float32, device=dev)
+    d.plan(d_uvw, d_wl)
+    d.invert(d_vis, 1.0, out)
+    st = d.stats()
+    assert st["chunks"] > nw * (w // 32) ** 2 + 50  # extra work items exist
+    assert rel(out.cpu().numpy(), re) < 1e-6
+    rng = np.random.default_rng(3)
+    model = rng.standard_normal((2, w, w)).astype(np.float32)
+    pred = torch.empty_like(d_vis)
+    d.predict(torch.from_numpy(model).to(dev), None, pred)
+    v_o = oracle_predict(oracle, w, w, dl, dl, uvw_l, model[1], None, nw, 0.0, maxw)
+    v = pred.cpu().numpy()[:, :, 1].reshape(-1)
+    ok = ~np.isnan(v_o)
+    assert np.array_equal(np.isnan(v), ~ok)
+    assert rel(v[ok], v_o[ok]) < TOL_IMG[precision]

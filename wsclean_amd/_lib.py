@@ -33,6 +33,7 @@ class HwsConfig(C.Structure):
         ("n_images", C.c_size_t), ("complex_mask", C.c_uint32),
         ("l_shift", C.c_double), ("m_shift", C.c_double),
         ("rotate_visibilities", C.c_int), ("tile_size", C.c_size_t),
+        ("work_chunk", C.c_size_t),
     ]
 
 

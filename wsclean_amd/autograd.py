@@ -45,8 +45,8 @@ class WStackingOperator:
         self.g = DeviceGridder(width, height, pixel_size_x, pixel_size_y, kernel_size,
                                oversampling, kernel_mode=kernel_mode, precision=precision,
                                device=self.device.index or 0, n_images=1, l_shift=l_shift,
-                               m_shift=m_shift, tile_size=tile_size)
-        self.g.set_stream(torch.cuda.current_stream(self.device))
+                               m_shift=m_shift, tile_size=tile_size,
+                               stream=torch.cuda.current_stream(self.device))
         self.n_passes = self.g.prepare_w_layers(n_w_layers, *w_limits)
         self.g.plan(self.uvw, self.wl)
 

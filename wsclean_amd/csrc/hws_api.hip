@@ -273,6 +273,8 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
     return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "n_images must be 1..4");
   if (c->tile_size != 0 && (c->tile_size < 8 || c->tile_size > 64))
     return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "tile_size must be 0 or 8..64");
+  if (c->work_chunk > (size_t(1) << 30))
+    return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "work_chunk must be at most 2^30");
   if (!(c->pixel_size_x > 0.0) || !(c->pixel_size_y > 0.0))
     return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "pixel sizes must be positive");
 
@@ -375,7 +377,7 @@ void hws_destroy(hws_gridder* g) {
   if (!g) return;
   if (g->cfg.device >= 0) (void)hipSetDevice(g->cfg.device);
   if (g->ownStream) (void)hipStreamSynchronize(g->ownStream);
-  if (g->stream && g->stream != g->ownStream) (void)hipStreamSynchronize(g->stream);
+  if (g->userStream) (void)hipStreamSynchronize(g->stream);
   freeDev(g->dKernel);
   freeDev(g->dKernelPair);
   freeDev(g->dCorrXf);
@@ -402,6 +404,8 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.recSorted);
     freeDev(kp.counts);
     freeDev(kp.binStart);
+    freeDev(kp.extraStart);
+    freeDev(kp.partials);
   }
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
@@ -411,7 +415,9 @@ void hws_destroy(hws_gridder* g) {
 
 hws_status hws_set_stream(hws_gridder* g, void* hip_stream) {
   if (!g) return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "null context");
-  g->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : g->ownStream;
+  // NULL is HIP's null stream, as everywhere in HIP
+  g->stream = static_cast<hipStream_t>(hip_stream);
+  g->userStream = true;
   return HWS_OK;
 }
 
@@ -832,12 +838,17 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
     s.ms_total = total;
   }
   if (g->plan.counters) {
-    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
-    HWS_TRY(checkHip(g, hipMemcpy(c, g->plan.counters, sizeof(c), hipMemcpyDeviceToHost), "counters"));
+    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HWS_TRY(checkHip(g, hipMemcpyAsync(c, g->plan.counters, sizeof(c), hipMemcpyDeviceToHost,
+                                       g->stream), "counters"));
+    HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "counters"));
     // summed over the image kinds present (each kind places every sample)
     s.samples_gridded = c[0] + c[3];
     s.samples_off_grid = c[1] + c[4];
     s.samples_off_layer = c[2] + c[5];
+    s.bins_used = c[6] + c[7];
+    for (const auto& kp : g->plan.kinds)
+      if (kp.active) s.chunks += g->plan.nbins + kp.nExtra;
   }
   *out = s;
   return HWS_OK;

@@ -196,6 +196,10 @@ hws_status validateGeometry(size_t width, size_t height, double dl, double dm, s
   if (width == 0 || height == 0 || width > 65536 || height > 65536 ||
       width * height > (size_t(1) << 31))
     return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "image size out of range");
+  // WSClean rounds odd image sizes up to even ones (main/settings.cpp:358-367);
+  // its pixel-to-(l, m) remap assumes even sizes
+  if (width % 2 != 0 || height % 2 != 0)
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "image width and height must be even");
   if (!(dl > 0.0) || !(dm > 0.0))
     return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "pixel sizes must be positive");
   if (nchan == 0 || nchan > (size_t(1) << 20) || stride == 0)

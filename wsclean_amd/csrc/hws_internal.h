@@ -67,6 +67,13 @@ struct KindPlan {
   uint2* recSorted = nullptr;    // records ordered by bin [capacity]
   uint32_t* counts = nullptr;    // histogram, then scatter cursor [nbins + 1]
   uint32_t* binStart = nullptr;  // exclusive scan of counts [nbins + 1]
+  // A bin with more than `chunk` records is split into work items of at most
+  // `chunk` records: item 0 writes the tile, items 1.. ("extra" items) leave
+  // their tiles in `partials`, summed into the layer afterwards.
+  uint32_t* extraStart = nullptr;  // exclusive scan of extra items per bin [nbins + 1]
+  float2* partials = nullptr;      // [nExtra][nImg][T * T]
+  uint64_t nExtra = 0;
+  size_t extraCapacity = 0, partialCapacity = 0;
   size_t countsCapacity = 0, capacity = 0, binCapacity = 0;
 };
 
@@ -77,12 +84,13 @@ struct Plan {
   const double* wavelengths = nullptr;  // device
   int tilesX = 0, tilesY = 0, tileSize = 0;
   size_t nbins = 0;
+  uint32_t chunk = 0;  // records per work item of the binned kernels
   KindPlan kinds[2];
   // temporaries
   uint32_t* scanTmp = nullptr;
   uint32_t* scanScratch = nullptr;
   size_t scanCapacity = 0, scanScratchCapacity = 0;
-  // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}]
+  // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}], [6 + kind]: bins used
   unsigned long long* counters = nullptr;
 };
 
@@ -95,6 +103,7 @@ struct hws_gridder {
   bool isComplex[hws::kMaxImages] = {};
   bool anyReal = false, anyComplex = false;
   hipStream_t ownStream = nullptr, stream = nullptr;
+  bool userStream = false;  // stream was set by hws_set_stream (may be the null stream)
   std::string err;
 
   // kernels

@@ -151,10 +151,11 @@ __device__ __forceinline__ Placement place(const GeomArgs& ga,
   return p;
 }
 
-__device__ __forceinline__ void waveCount(unsigned long long* counter,
-                                          bool pred) {
+// Adds the number of lanes of the wave with `pred` to a workgroup counter in
+// LDS (one atomic per wave; the workgroup flushes once to memory).
+__device__ __forceinline__ void waveCount(unsigned long long* ldsCounter, bool pred) {
   const unsigned long long mask = __ballot(pred);
-  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(counter, (unsigned long long)__popcll(mask));
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(ldsCounter, (unsigned long long)__popcll(mask));
 }
 
 // Consecutive lanes of a wave whose samples fall in the same bin form runs
@@ -255,38 +256,115 @@ __device__ __forceinline__ Placement placeSample(const GeomArgs& ga, const Layer
 // the (layer, tile) bins of all tiles its footprint touches. Sample
 // i = row * nchan + channel, the order of the visibility array; the lanes of a
 // run of consecutive samples in one bin add their count with one atomic.
+// Pass 2 (SCATTER) repeats the placement and writes the records to their
+// bins. Workgroups stride over the samples; the gridded / off-grid /
+// off-layer counters are summed per workgroup and added to memory once
+// (a same-address atomic per wave serialises in L2).
 template <bool SCATTER>
 __global__ void __launch_bounds__(kThreads)
 binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
           int nchan, GeomArgs ga, KindArgs k) {
+  __shared__ unsigned long long sCount[3];
+  if (!SCATTER && threadIdx.x < 3) sCount[threadIdx.x] = 0ull;
+  __syncthreads();
   const size_t n = nrow * size_t(nchan);
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  Placement p;
-  p.layer = -1;
-  p.onGrid = 0;
-  if (i < n) p = placeSample(ga, k.lm, uvw, wl, i, nchan);
-  const bool ok = i < n && p.layer >= 0 && p.onGrid;
-  if (!SCATTER) {
-    waveCount(&k.counters[0], ok);
-    waveCount(&k.counters[1], i < n && p.layer >= 0 && !p.onGrid);
-    waveCount(&k.counters[2], i < n && p.layer < 0);
-  }
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    uint32_t bin = 0xFFFFFFFFu;
-    uint2 rec = make_uint2(0u, 0u);
-    const bool has = ok && emission(ga, p, e, uint32_t(i), bin, rec);
-    const LaneRun r = laneRun(bin, has);
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t base = size_t(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const size_t i = base + threadIdx.x;
+    Placement p;
+    p.layer = -1;
+    p.onGrid = 0;
+    if (i < n) p = placeSample(ga, k.lm, uvw, wl, i, nchan);
+    const bool ok = i < n && p.layer >= 0 && p.onGrid;
     if (!SCATTER) {
-      if (has && r.head == lane) atomicAdd(&k.counts[bin], uint32_t(r.len));
-    } else {
-      uint32_t base = 0;
-      if (has && r.head == lane) base = atomicAdd(&k.cursor[bin], uint32_t(r.len));
-      base = __shfl(base, r.head);
-      if (has) k.recSorted[base + uint32_t(lane - r.head)] = rec;
+      waveCount(&sCount[0], ok);
+      waveCount(&sCount[1], i < n && p.layer >= 0 && !p.onGrid);
+      waveCount(&sCount[2], i < n && p.layer < 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint32_t bin = 0xFFFFFFFFu;
+      uint2 rec = make_uint2(0u, 0u);
+      const bool has = ok && emission(ga, p, e, uint32_t(i), bin, rec);
+      const LaneRun r = laneRun(bin, has);
+      if (!SCATTER) {
+        if (has && r.head == lane) atomicAdd(&k.counts[bin], uint32_t(r.len));
+      } else {
+        uint32_t pos = 0;
+        if (has && r.head == lane) pos = atomicAdd(&k.cursor[bin], uint32_t(r.len));
+        pos = __shfl(pos, r.head);
+        if (has) k.recSorted[pos + uint32_t(lane - r.head)] = rec;
+      }
     }
   }
+  if (!SCATTER) {
+    __syncthreads();
+    if (threadIdx.x < 3 && sCount[threadIdx.x])
+      atomicAdd(&k.counters[threadIdx.x], sCount[threadIdx.x]);
+  }
+}
+
+// Extra work items per bin: a bin of c records is processed as
+// ceil(c / chunk) items, the first plus (c - 1) / chunk extra ones. Also
+// counts the non-empty bins.
+__global__ void __launch_bounds__(kThreads)
+extraItemsKernel(const uint32_t* __restrict__ binStart, size_t nbins, uint32_t chunk,
+                 uint32_t* __restrict__ extra, unsigned long long* __restrict__ nonEmpty) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t c = 0;
+  if (b < nbins) {
+    c = binStart[b + 1] - binStart[b];
+    extra[b] = c > chunk ? (c - 1u) / chunk : 0u;
+  }
+  const int cnt = __syncthreads_count(b < nbins && c > 0u);
+  if (threadIdx.x == 0 && cnt) atomicAdd(nonEmpty, (unsigned long long)cnt);
+}
+
+// One work item of a pass: bin, item index within the bin (0 = the item
+// that writes the tile) and its record range. Items [0, nb) are the bins
+// binLo.. in order; items nb.. are the extra items of the pass's bins in
+// bin order, found by binary search of extraStart.
+struct WorkItem {
+  uint32_t bin, j, start, count;
+};
+
+__device__ __forceinline__ uint32_t binOfExtra(const uint32_t* __restrict__ extraStart,
+                                               uint32_t lo, uint32_t hi, uint32_t e) {
+  // largest b in [lo, hi) with extraStart[b] <= e; then e < extraStart[b + 1]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (extraStart[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ WorkItem decodeItem(uint32_t item, uint32_t binLo, uint32_t binHi,
+                                               const uint32_t* __restrict__ binStart,
+                                               const uint32_t* __restrict__ extraStart,
+                                               uint32_t chunk) {
+  WorkItem w;
+  const uint32_t nb = binHi - binLo;
+  if (item < nb) {
+    w.bin = binLo + item;
+    w.j = 0;
+  } else {
+    const uint32_t e = extraStart[binLo] + (item - nb);
+    w.bin = binOfExtra(extraStart, binLo, binHi, e);
+    w.j = e - extraStart[w.bin] + 1u;
+  }
+  const uint32_t s0 = binStart[w.bin], s1 = binStart[w.bin + 1];
+  w.start = s0 + w.j * chunk;
+  w.count = min(s1 - w.start, chunk);
+  return w;
+}
+
+__device__ __forceinline__ uint32_t passItems(uint32_t binLo, uint32_t binHi,
+                                              const uint32_t* __restrict__ extraStart) {
+  return (binHi - binLo) + (extraStart[binHi] - extraStart[binLo]);
 }
 
 
@@ -362,6 +440,7 @@ struct GridArgs {
   int tilesPerLayer;
   int sg;            // degrid subgrid edge = tileSize + K - 1
   size_t nchan;
+  uint32_t chunk;    // records per work item
 };
 
 __device__ __forceinline__ float2 rotateSample(float2 v, const GridArgs& a,
@@ -403,12 +482,14 @@ __device__ __forceinline__ int readLaneI(int v, int lane) {
 
 constexpr int kGridThreads = 512;
 
-// Gridding of one image kind, one (layer, uv tile) per workgroup at a time:
-// the workgroup owns the T x T tile, accumulates every record binned to it
-// in an LDS tile, and writes the tile to the layer once with plain stores
-// (ACCUM adds to what the layer holds, for callers that grid a pass in
-// several batches). No global atomics: tiles without samples are written
-// as zeros, which also clears the layer.
+// Gridding of one image kind, one work item per workgroup at a time: the
+// workgroup owns the T x T tile of the item's (layer, uv tile) bin,
+// accumulates the item's records in an LDS tile, and writes the tile once
+// with plain stores: item 0 of a bin to the layer (ACCUM adds to what the
+// layer holds, for callers that grid a pass in several batches), extra items
+// of a large bin to their partial tile, which reducePartialsKernel adds to
+// the layer afterwards in a fixed order. No global atomics: tiles without
+// samples are written as zeros, which also clears the layer.
 // Each wave takes its records 64 at a time: lane l loads record l and its
 // visibilities (latency paid once per 64), then the wave walks them with the
 // values broadcast by readlane; lane t owns tap t of the K x K footprint (TG
@@ -419,11 +500,13 @@ constexpr int kGridThreads = 512;
 template <int NIMG, int TG, bool ACCUM>
 __global__ void __launch_bounds__(kGridThreads)
 gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
-                const uint2* __restrict__ recs, const float2* __restrict__ vis,
-                const float* __restrict__ ktab, const __half2* __restrict__ khalf,
-                const double* __restrict__ uvw, const double* __restrict__ wl,
-                float2* __restrict__ layers) {
+                const uint32_t* __restrict__ extraStart, const uint2* __restrict__ recs,
+                const float2* __restrict__ vis, const float* __restrict__ ktab,
+                const __half2* __restrict__ khalf, const double* __restrict__ uvw,
+                const double* __restrict__ wl, float2* __restrict__ layers,
+                float2* __restrict__ partials) {
   extern __shared__ float smem[];
+  __shared__ WorkItem sItem;
   const int K = a.ga.K, OS = a.ga.OS, T = a.ga.tileSize, T2 = T * T;
   float* kt = smem;                            // [OS][K]
   float* tile = smem + ((OS * K + 3) & ~3);    // [NIMG*2][T*T]
@@ -443,15 +526,19 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
     ti[q] = tapOk[q] ? t - tj[q] * K : 0;
   }
   const size_t HW = size_t(a.ga.W) * a.ga.H;
+  const uint32_t nItems = passItems(binLo, binHi, extraStart);
   __syncthreads();
-  for (uint32_t bin = binLo + blockIdx.x; bin < binHi; bin += gridDim.x) {
-    const uint32_t start = binStart[bin], count = binStart[bin + 1] - start;
+  for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
+    if (threadIdx.x == 0) sItem = decodeItem(item, binLo, binHi, binStart, extraStart, a.chunk);
+    __syncthreads();
+    const WorkItem wi = sItem;
+    const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
-    if (ACCUM && count == 0) continue;  // block-uniform
     for (int i = threadIdx.x; i < NIMG * 2 * T2; i += blockDim.x) tile[i] = 0.0f;
-    __syncthreads();
+    __syncthreads();  // also: every thread has read sItem
+    if (ACCUM && count == 0) continue;  // block-uniform; nothing to add
     for (uint32_t b0 = uint32_t(wave) * 64u; b0 < count; b0 += uint32_t(nwaves) * 64u) {
       const uint32_t it = b0 + uint32_t(lane);
       int key = 0, kxo = 0, kyo = 0;
@@ -537,6 +624,16 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
       }
     }
     __syncthreads();
+    if (wi.j > 0) {
+      // an extra item of a large bin: its tile goes to the partial slot
+      float2* dst = partials + size_t(extraStart[bin] + wi.j - 1u) * size_t(NIMG * T2);
+      for (int c = threadIdx.x; c < T2; c += blockDim.x)
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi)
+          dst[gi * T2 + c] = make_float2(tile[(2 * gi) * T2 + c], tile[(2 * gi + 1) * T2 + c]);
+      __syncthreads();
+      continue;
+    }
     // write the tile in layer storage order (blocked: 4 rows x T columns
     // contiguous per row block)
     const int x0 = tx * T, y0 = ty * T;
@@ -569,13 +666,15 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   }
 }
 
-// Degridding of one image kind: per (layer, tile) the (T + K - 1)^2 subgrid
-// is gathered from the layer into LDS once; each lane then sums the K x K
-// taps of one primary record (the record of the tile holding the footprint
-// start) and writes its visibilities.
+// Degridding of one image kind, one work item per workgroup at a time: the
+// (T + K - 1)^2 subgrid around the item's (layer, tile) is gathered from the
+// layer into LDS; each lane then sums the K x K taps of one primary record
+// (the record of the tile holding the footprint start) and writes its
+// visibilities.
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
 degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
+                  const uint32_t* __restrict__ extraStart,
                   const uint2* __restrict__ recs, const float* __restrict__ ktab,
                   const __half2* __restrict__ khalf, const double* __restrict__ uvw,
                   const double* __restrict__ wl, const float2* __restrict__ layers,
@@ -588,8 +687,11 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const size_t HW = size_t(a.ga.W) * a.ga.H;
-  for (uint32_t bin = binLo + blockIdx.x; bin < binHi; bin += gridDim.x) {
-    const uint32_t start = binStart[bin], count = binStart[bin + 1] - start;
+  const uint32_t nItems = passItems(binLo, binHi, extraStart);
+  for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
+    // every lane decodes the item (uniform loads)
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, extraStart, a.chunk);
+    const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
     if (count == 0) continue;  // block-uniform
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
@@ -642,6 +744,46 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   }
 }
 
+
+// Adds the partial tiles of the extra items of every split bin of a pass to
+// the layer, in item order. One workgroup per split bin: the workgroup that
+// meets the bin's first extra item does the bin.
+template <int NIMG>
+__global__ void __launch_bounds__(kThreads)
+reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
+                     const float2* __restrict__ partials, float2* __restrict__ layers) {
+  const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
+  const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
+  const uint32_t eLo = extraStart[binLo], eHi = extraStart[binHi];
+  const int T = a.ga.tileSize, T2 = T * T;
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  for (uint32_t e = eLo + blockIdx.x; e < eHi; e += gridDim.x) {
+    const uint32_t bin = binOfExtra(extraStart, binLo, binHi, e);
+    const uint32_t first = extraStart[bin];
+    if (e != first) continue;  // block-uniform
+    const uint32_t nEx = extraStart[bin + 1] - first;
+    const int layer = int(bin / uint32_t(a.tilesPerLayer));
+    const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
+    const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+    for (int c = threadIdx.x; c < T2; c += blockDim.x) {
+      const int cy = c / T, cx = c - cy * T;
+      const size_t off = cellIndex(tx * T + cx, ty * T + cy, a.ga.W, a.ga.blocked);
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        float2 sum = make_float2(0.0f, 0.0f);
+        for (uint32_t q = 0; q < nEx; ++q) {
+          const float2 v = partials[(size_t(first + q) * NIMG + gi) * T2 + c];
+          sum.x += v.x;
+          sum.y += v.y;
+        }
+        float2* dst = layers +
+                      (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
+        const float2 o = *dst;
+        *dst = make_float2(o.x + sum.x, o.y + sum.y);
+      }
+    }
+  }
+}
 
 // ----------------------------------------------------- plain per-channel
 struct PlainArgs {
@@ -1137,6 +1279,17 @@ int persistentBlocks(const hws_gridder* g, int perCu) {
   return std::max(1, cus) * perCu;
 }
 
+// Records per work item: about a quarter of an even share of the records
+// (about 1.2 per sample) over the grid kernel's workgroups, a power of two
+// in [1024, 65536]; or the configured value.
+uint32_t workChunk(const hws_gridder* g, size_t nSamples) {
+  if (g->cfg.work_chunk) return uint32_t(g->cfg.work_chunk);
+  const size_t target = (nSamples + nSamples / 4) / (size_t(persistentBlocks(g, 4)) * 4) + 1;
+  size_t c = 1024;
+  while (c < target && c < 65536) c <<= 1;
+  return uint32_t(c);
+}
+
 }  // namespace
 
 hws_status launchFill(hws_gridder* g, void* ptr, uint32_t value, size_t n32) {
@@ -1173,6 +1326,9 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
   HWS_TRY(launchFill(g, pl.counters, 0u, 16));
 
   const GeomArgs ga = geomArgs(g);
+  pl.chunk = workChunk(g, n);
+  const unsigned binBlocks =
+      unsigned(std::max<size_t>(1, std::min<size_t>(blocksFor(n), size_t(persistentBlocks(g, 16)))));
   for (int kind = 0; kind < 2; ++kind) {
     KindPlan& kp = pl.kinds[kind];
     kp.kind = kind;
@@ -1183,28 +1339,43 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     if (!kp.active) continue;
     HWS_TRY(ensureAlloc(g, kp.counts, kp.countsCapacity, pl.nbins + 1, "plan counts"));
     HWS_TRY(ensureAlloc(g, kp.binStart, kp.binCapacity, pl.nbins + 1, "plan bin offsets"));
+    HWS_TRY(ensureAlloc(g, kp.extraStart, kp.extraCapacity, pl.nbins + 1, "plan item offsets"));
     HWS_TRY(launchFill(g, kp.counts, 0u, pl.nbins + 1));
     KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind};
     if (n > 0) {
-      hipLaunchKernelGGL(binKernel<false>, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
+      hipLaunchKernelGGL(binKernel<false>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
                          dUvw, dWl, nrow, int(nchan), ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin count launch"));
     }
     HWS_TRY(exclusiveScan(g, kp.counts, kp.binStart, pl.nbins, pl.scanScratch));
-    // the record total is needed to size the sorted array
-    uint32_t total = 0;
-    HWS_TRY(checkHip(g, hipMemcpyAsync(&total, kp.binStart + pl.nbins, sizeof(uint32_t),
+    // work items: the counts array is free until the scatter cursor is set
+    hipLaunchKernelGGL(extraItemsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0,
+                       g->stream, kp.binStart, pl.nbins, pl.chunk, kp.counts,
+                       pl.counters + 6 + kind);
+    HWS_TRY(checkHip(g, hipGetLastError(), "work item launch"));
+    HWS_TRY(exclusiveScan(g, kp.counts, kp.extraStart, pl.nbins, pl.scanScratch));
+    // the record and extra-item totals size the sorted array and partials
+    uint32_t totals[2] = {0, 0};
+    HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[0], kp.binStart + pl.nbins, sizeof(uint32_t),
                                        hipMemcpyDeviceToHost, g->stream), "record total"));
-    HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "record total"));
+    HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[1], kp.extraStart + pl.nbins, sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, g->stream), "item total"));
+    HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "plan totals"));
+    const uint32_t total = totals[0];
     kp.nItems = total;
+    kp.nExtra = totals[1];
     HWS_TRY(ensureAlloc(g, kp.recSorted, kp.capacity, size_t(total), "plan sorted records"));
+    if (kp.nExtra)
+      HWS_TRY(ensureAlloc(g, kp.partials, kp.partialCapacity,
+                          size_t(kp.nExtra) * kp.nImg * pl.tileSize * pl.tileSize,
+                          "plan partial tiles"));
     // cursor = bin offsets
     HWS_TRY(checkHip(g, hipMemcpyAsync(kp.counts, kp.binStart, pl.nbins * sizeof(uint32_t),
                                        hipMemcpyDeviceToDevice, g->stream), "cursor copy"));
     ka.cursor = kp.counts;
     ka.recSorted = kp.recSorted;
     if (n > 0) {
-      hipLaunchKernelGGL(binKernel<true>, dim3(blocksFor(n)), dim3(kThreads), 0, g->stream,
+      hipLaunchKernelGGL(binKernel<true>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
                          dUvw, dWl, nrow, int(nchan), ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin scatter launch"));
     }
@@ -1226,6 +1397,7 @@ GridArgs gridArgs(const hws_gridder* g, int pass) {
   a.tilesPerLayer = pl.tilesX * pl.tilesY;
   a.sg = pl.tileSize + g->K - 1;
   a.nchan = pl.nchan;
+  a.chunk = pl.chunk;
   return a;
 }
 }  // namespace
@@ -1246,8 +1418,9 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
     const float2* vis = reinterpret_cast<const float2*>(dVis);
 #define HWS_GRID_LAUNCH(N, T, ACC)                                                           \
   hipLaunchKernelGGL((gridTilesKernel<N, T, ACC>), dim3(blocks), dim3(kGridThreads), lds,    \
-                     g->stream, a, kp.binStart, kp.recSorted, vis, g->dKernel, khalf, pl.uvw, \
-                     pl.wavelengths, reinterpret_cast<float2*>(g->dLayers))
+                     g->stream, a, kp.binStart, kp.extraStart, kp.recSorted, vis, g->dKernel, \
+                     khalf, pl.uvw, pl.wavelengths, reinterpret_cast<float2*>(g->dLayers),     \
+                     kp.partials)
 #define HWS_GRID_TG(N, ACC)          \
   if (tg == 1)                       \
     HWS_GRID_LAUNCH(N, 1, ACC);      \
@@ -1275,6 +1448,23 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
 #undef HWS_GRID_TG
 #undef HWS_GRID_LAUNCH
     HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
+    if (kp.nExtra) {
+      const unsigned rblocks = unsigned(std::min<uint64_t>(kp.nExtra, 4096));
+      float2* layers = reinterpret_cast<float2*>(g->dLayers);
+      switch (kp.nImg) {
+#define HWS_REDUCE_CASE(N)                                                                    \
+  case N:                                                                                     \
+    hipLaunchKernelGGL(reducePartialsKernel<N>, dim3(rblocks), dim3(kThreads), 0, g->stream,   \
+                       a, kp.extraStart, kp.partials, layers);                                \
+    break;
+        HWS_REDUCE_CASE(1)
+        HWS_REDUCE_CASE(2)
+        HWS_REDUCE_CASE(3)
+        HWS_REDUCE_CASE(4)
+#undef HWS_REDUCE_CASE
+      }
+      HWS_TRY(checkHip(g, hipGetLastError(), "partial reduce launch"));
+    }
   }
   return HWS_OK;
 }
@@ -1296,7 +1486,7 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
 #define HWS_DEGRID_CASE(N)                                                                 \
   case N:                                                                                  \
     hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream,  \
-                       a, kp.binStart, kp.recSorted, g->dKernel, khalf, pl.uvw,            \
+                       a, kp.binStart, kp.extraStart, kp.recSorted, g->dKernel, khalf, pl.uvw, \
                        pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out);  \
     break;
       HWS_DEGRID_CASE(1)

@@ -23,12 +23,12 @@ def _p(a, t):
 
 
 def _cfg(width, height, dl, dm, kernel_size, oversampling, kernel_mode, precision,
-         device, n_images, complex_mask, shift, rotate, tile_size):
+         device, n_images, complex_mask, shift, rotate, tile_size, work_chunk=0):
     mode = KERNEL_MODES[kernel_mode] if isinstance(kernel_mode, str) else int(kernel_mode)
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     return HwsConfig(width, height, dl, dm, kernel_size, oversampling, mode, prec, device,
                      n_images, complex_mask, shift[0], shift[1], int(bool(rotate)),
-                     tile_size)
+                     tile_size, work_chunk)
 
 
 class _Context:
@@ -73,13 +73,14 @@ class WStackingGridder(_Context):
     def __init__(self, width, height, pixel_size_x, pixel_size_y, kernel_size=7,
                  oversampling=1023, *, kernel_mode="kb", precision="reference", device=0,
                  n_images=1, complex_images=(), l_shift=0.0, m_shift=0.0,
-                 rotate_visibilities=False, tile_size=0):
+                 rotate_visibilities=False, tile_size=0, work_chunk=0):
         mask = 0
         for i in complex_images:
             mask |= 1 << i
         super().__init__(_cfg(width, height, pixel_size_x, pixel_size_y, kernel_size,
                               oversampling, kernel_mode, precision, device, n_images, mask,
-                              (l_shift, m_shift), rotate_visibilities, tile_size))
+                              (l_shift, m_shift), rotate_visibilities, tile_size,
+                              work_chunk))
         self.width, self.height, self.n_images = width, height, n_images
         self.complex_images = tuple(complex_images)
         self.n_passes = 0
@@ -173,7 +174,8 @@ def _ptr(t):
 
 
 class DeviceGridder(_Context):
-    """Batched one-call invert/predict on torch CUDA (HIP) tensors.
+    """Batched one-call invert/predict on torch CUDA (HIP) tensors, ordered on
+    torch's current stream of `device` at construction (or on `stream`).
 
     uvw_m: float64 [nrow, 3]; wavelengths: float64 [nchan];
     vis: complex64 [nrow, nchan, n_images]; images: float32 [n_images, H, W].
@@ -182,20 +184,25 @@ class DeviceGridder(_Context):
     def __init__(self, width, height, pixel_size_x, pixel_size_y, kernel_size=7,
                  oversampling=63, *, kernel_mode="kb", precision="mixed", device=0,
                  n_images=1, complex_images=(), l_shift=0.0, m_shift=0.0,
-                 rotate_visibilities=True, tile_size=0, stream=None):
+                 rotate_visibilities=True, tile_size=0, stream=None, work_chunk=0):
         mask = 0
         for i in complex_images:
             mask |= 1 << i
         super().__init__(_cfg(width, height, pixel_size_x, pixel_size_y, kernel_size,
                               oversampling, kernel_mode, precision, device, n_images, mask,
-                              (l_shift, m_shift), rotate_visibilities, tile_size))
+                              (l_shift, m_shift), rotate_visibilities, tile_size,
+                              work_chunk))
         self.width, self.height, self.n_images = width, height, n_images
         self.complex_images = tuple(complex_images)
         self.device = device
         self.n_passes = 0
         self.nrow = self.nchan = 0
-        if stream is not None:
-            self.set_stream(stream)
+        # order the gridder's work with torch's on the same stream unless the
+        # caller names another one
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(torch.device("cuda", device))
+        self.set_stream(stream)
 
     def set_stream(self, stream):
         self._check(self.L.hws_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
