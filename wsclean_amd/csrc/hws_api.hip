@@ -405,8 +405,13 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.counts);
     freeDev(kp.binStart);
     freeDev(kp.extraStart);
+    freeDev(kp.itemStart);
     freeDev(kp.partials);
+    freeDev(kp.keys);
+    freeDev(kp.keysAlt);
+    freeDev(kp.recAlt);
   }
+  if (pl.sortTemp) (void)hipFree(pl.sortTemp);
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
   if (g->ownStream) (void)hipStreamDestroy(g->ownStream);
@@ -720,7 +725,7 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
       g->layersZero = false;
       HWS_TRY(gridPlain(g, pl.nrow, pl.nchan, pl.uvw, pl.wavelengths, d_vis, pass));
     } else {
-      // every tile of the pass's layers is written, empty ones as zeros
+      // the pass's layers are cleared and the non-empty tiles written
       g->layersZero = false;
       HWS_TRY(gridBinned(g, d_vis, pass, false));
     }
@@ -848,7 +853,7 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
     s.samples_off_layer = c[2] + c[5];
     s.bins_used = c[6] + c[7];
     for (const auto& kp : g->plan.kinds)
-      if (kp.active) s.chunks += g->plan.nbins + kp.nExtra;
+      if (kp.active) s.chunks += kp.nWorkItems;
   }
   *out = s;
   return HWS_OK;

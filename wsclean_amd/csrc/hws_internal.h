@@ -70,7 +70,17 @@ struct KindPlan {
   // A bin with more than `chunk` records is split into work items of at most
   // `chunk` records: item 0 writes the tile, items 1.. ("extra" items) leave
   // their tiles in `partials`, summed into the layer afterwards.
+  // records in bin order are radix-sorted by (bin, footprint origin) so that
+  // records sharing a footprint are consecutive: sort keys and the second
+  // buffers of the sort
+  uint32_t* keys = nullptr;
+  uint32_t* keysAlt = nullptr;
+  uint2* recAlt = nullptr;
+  size_t keyCapacity = 0, keyAltCapacity = 0, altCapacity = 0;
   uint32_t* extraStart = nullptr;  // exclusive scan of extra items per bin [nbins + 1]
+  uint32_t* itemStart = nullptr;   // exclusive scan of all items per bin [nbins + 1]
+  uint64_t nWorkItems = 0;
+  size_t itemCapacity = 0;
   float2* partials = nullptr;      // [nExtra][nImg][T * T]
   uint64_t nExtra = 0;
   size_t extraCapacity = 0, partialCapacity = 0;
@@ -92,6 +102,8 @@ struct Plan {
   size_t scanCapacity = 0, scanScratchCapacity = 0;
   // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}], [6 + kind]: bins used
   unsigned long long* counters = nullptr;
+  void* sortTemp = nullptr;  // radix sort scratch
+  size_t sortTempCapacity = 0;
 };
 
 }  // namespace hws
@@ -168,6 +180,11 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl);
 hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate);
 hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass);
+// Sorts n records (and their 32-bit keys) by key bits [0, endBit) on the
+// context's stream; on return `recs` holds the sorted records (the two record
+// buffers may have been swapped).
+hws_status sortRecords(hws_gridder* g, uint32_t*& keys, uint32_t*& keysAlt, uint2*& recs,
+                       uint2*& recsAlt, size_t n, int endBit);
 hws_status gridPlain(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl, const float* dVis,
                      int pass);

@@ -91,6 +91,14 @@ __global__ void fillKernel(uint32_t* p, uint32_t value, size_t n) {
     p[i] = value;
 }
 
+// Zeroes n 16-byte words (the layers of a pass before gridding).
+__global__ void zero16Kernel(uint4* p, size_t n) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = z;
+}
+
 // ------------------------------------------------------------- binning
 // Layer index as WToLayer computes it (gridding/wstackinggridder.h:182-193);
 // returns -1 when the rounded value is negative or >= nW.
@@ -191,6 +199,7 @@ struct KindArgs {
   uint32_t* cursor;              // scatter: next free position per bin
   uint2* recSorted;              // scatter: records ordered by bin
   unsigned long long* counters;  // gridded, off grid, off layer
+  uint32_t* sortKeys;            // scatter: bin << 12 | footprint origin, or null
 };
 
 // The uv tiles a sample's footprint touches. Footprint cells run from
@@ -294,7 +303,10 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t 
         uint32_t pos = 0;
         if (has && r.head == lane) pos = atomicAdd(&k.cursor[bin], uint32_t(r.len));
         pos = __shfl(pos, r.head);
-        if (has) k.recSorted[pos + uint32_t(lane - r.head)] = rec;
+        if (has) {
+          k.recSorted[pos + uint32_t(lane - r.head)] = rec;
+          if (k.sortKeys) k.sortKeys[pos + uint32_t(lane - r.head)] = (bin << 12) | (rec.y & 4095u);
+        }
       }
     }
   }
@@ -305,9 +317,11 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t 
   }
 }
 
-// Extra work items per bin: a bin of c records is processed as
-// ceil(c / chunk) items, the first plus (c - 1) / chunk extra ones. Also
-// counts the non-empty bins.
+// Work items per bin: a bin of c > 0 records is processed as ceil(c / chunk)
+// items, the first plus (c - 1) / chunk extra ones; an empty bin has none
+// (its tile is cleared before gridding). Pass 1 writes the extra items per bin
+// (and counts the non-empty bins), pass 2 -- after the scan of pass 1 into
+// extraStart -- all items per bin.
 __global__ void __launch_bounds__(kThreads)
 extraItemsKernel(const uint32_t* __restrict__ binStart, size_t nbins, uint32_t chunk,
                  uint32_t* __restrict__ extra, unsigned long long* __restrict__ nonEmpty) {
@@ -321,20 +335,30 @@ extraItemsKernel(const uint32_t* __restrict__ binStart, size_t nbins, uint32_t c
   if (threadIdx.x == 0 && cnt) atomicAdd(nonEmpty, (unsigned long long)cnt);
 }
 
+__global__ void __launch_bounds__(kThreads)
+binItemsKernel(const uint32_t* __restrict__ binStart, const uint32_t* __restrict__ extraStart,
+               size_t nbins, uint32_t* __restrict__ items) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b < nbins)
+    items[b] = (binStart[b + 1] > binStart[b] ? 1u : 0u) + (extraStart[b + 1] - extraStart[b]);
+}
+
 // One work item of a pass: bin, item index within the bin (0 = the item
-// that writes the tile) and its record range. Items [0, nb) are the bins
-// binLo.. in order; items nb.. are the extra items of the pass's bins in
-// bin order, found by binary search of extraStart.
+// that writes the tile) and its record range. The items of the pass's bins
+// [binLo, binHi) are numbered in bin order from itemStart[binLo]; the bin of
+// an item is found by binary search of the offsets.
 struct WorkItem {
   uint32_t bin, j, start, count;
 };
 
-__device__ __forceinline__ uint32_t binOfExtra(const uint32_t* __restrict__ extraStart,
-                                               uint32_t lo, uint32_t hi, uint32_t e) {
-  // largest b in [lo, hi) with extraStart[b] <= e; then e < extraStart[b + 1]
+// Largest b in [lo, hi) with offs[b] <= v, for nondecreasing offs with
+// offs[lo] <= v < offs[hi]: the bin that owns entry v (bins owning no entry
+// are skipped because the next bin has the same offset).
+__device__ __forceinline__ uint32_t ownerOf(const uint32_t* __restrict__ offs, uint32_t lo,
+                                            uint32_t hi, uint32_t v) {
   while (hi - lo > 1u) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (extraStart[mid] <= e)
+    if (offs[mid] <= v)
       lo = mid;
     else
       hi = mid;
@@ -344,18 +368,12 @@ __device__ __forceinline__ uint32_t binOfExtra(const uint32_t* __restrict__ extr
 
 __device__ __forceinline__ WorkItem decodeItem(uint32_t item, uint32_t binLo, uint32_t binHi,
                                                const uint32_t* __restrict__ binStart,
-                                               const uint32_t* __restrict__ extraStart,
+                                               const uint32_t* __restrict__ itemStart,
                                                uint32_t chunk) {
   WorkItem w;
-  const uint32_t nb = binHi - binLo;
-  if (item < nb) {
-    w.bin = binLo + item;
-    w.j = 0;
-  } else {
-    const uint32_t e = extraStart[binLo] + (item - nb);
-    w.bin = binOfExtra(extraStart, binLo, binHi, e);
-    w.j = e - extraStart[w.bin] + 1u;
-  }
+  const uint32_t g = itemStart[binLo] + item;
+  w.bin = ownerOf(itemStart, binLo, binHi, g);
+  w.j = g - itemStart[w.bin];
   const uint32_t s0 = binStart[w.bin], s1 = binStart[w.bin + 1];
   w.start = s0 + w.j * chunk;
   w.count = min(s1 - w.start, chunk);
@@ -363,8 +381,8 @@ __device__ __forceinline__ WorkItem decodeItem(uint32_t item, uint32_t binLo, ui
 }
 
 __device__ __forceinline__ uint32_t passItems(uint32_t binLo, uint32_t binHi,
-                                              const uint32_t* __restrict__ extraStart) {
-  return (binHi - binLo) + (extraStart[binHi] - extraStart[binLo]);
+                                              const uint32_t* __restrict__ itemStart) {
+  return itemStart[binHi] - itemStart[binLo];
 }
 
 
@@ -473,48 +491,67 @@ __device__ __forceinline__ void loadKernelTable(float* kt, const float* __restri
   }
 }
 
-__device__ __forceinline__ float readLaneF(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-__device__ __forceinline__ int readLaneI(int v, int lane) {
-  return __builtin_amdgcn_readlane(v, lane);
+// Orders the LDS accesses of one wave before and after this point (lanes of
+// a wave exchanging data through LDS).
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kGridThreads = 512;
+constexpr int kGridWaves = 2;
+constexpr int kGridThreads = 64 * kGridWaves;
+
+// Row pitch of a wave's private tile: at least T, and congruent to +-7
+// modulo 32, so that the 7 rows of a 7 x 7 footprint fall on different
+// banks (at most two taps per bank).
+__host__ __device__ inline int privatePitch(int T) {
+  int p = T;
+  while ((p & 31) != 7 && (p & 31) != 25) ++p;
+  return p;
+}
 
 // Gridding of one image kind, one work item per workgroup at a time: the
-// workgroup owns the T x T tile of the item's (layer, uv tile) bin,
-// accumulates the item's records in an LDS tile, and writes the tile once
-// with plain stores: item 0 of a bin to the layer (ACCUM adds to what the
-// layer holds, for callers that grid a pass in several batches), extra items
-// of a large bin to their partial tile, which reducePartialsKernel adds to
-// the layer afterwards in a fixed order. No global atomics: tiles without
-// samples are written as zeros, which also clears the layer.
-// Each wave takes its records 64 at a time: lane l loads record l and its
-// visibilities (latency paid once per 64), then the wave walks them with the
-// values broadcast by readlane; lane t owns tap t of the K x K footprint (TG
-// groups of 64 taps for K > 8) and deposits only taps inside the tile.
-// While consecutive records keep the footprint in place (the channels of a
-// short baseline), each lane sums in registers and touches LDS only when the
-// footprint moves.
+// workgroup owns the T x T tile of the item's (layer, uv tile) bin. Each wave
+// takes a contiguous segment of the item's records and walks them one at a
+// time with lane t owning tap t of the K x K footprint (TG groups of 64 taps
+// for K > 8); the records' positions and visibilities are staged in LDS 64
+// at a time and read by every lane. While consecutive records keep the
+// footprint in place (neighbouring channels of a short baseline) each lane
+// sums in registers; when it moves, the lanes add their sums to the wave's
+// own copy of the tile in LDS with plain read-modify-write (the taps of one
+// footprint are distinct cells and no other wave writes the copy, so no
+// atomics are needed; LDS float atomics run far below LDS bandwidth). The
+// copies are then summed and the tile written once with plain stores:
+// item 0 of a bin to the layer (ACCUM adds to what the layer holds, for
+// callers that grid a pass in several batches), extra items of a large bin
+// to their partial tile, which reducePartialsKernel adds to the layer
+// afterwards in a fixed order. No global atomics: tiles without samples are
+// written as zeros, which also clears the layer.
 template <int NIMG, int TG, bool ACCUM>
 __global__ void __launch_bounds__(kGridThreads)
 gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
-                const uint32_t* __restrict__ extraStart, const uint2* __restrict__ recs,
+                const uint32_t* __restrict__ extraStart, const uint32_t* __restrict__ itemStart,
+                const uint2* __restrict__ recs,
                 const float2* __restrict__ vis, const float* __restrict__ ktab,
                 const __half2* __restrict__ khalf, const double* __restrict__ uvw,
                 const double* __restrict__ wl, float2* __restrict__ layers,
                 float2* __restrict__ partials) {
   extern __shared__ float smem[];
-  __shared__ WorkItem sItem;
   const int K = a.ga.K, OS = a.ga.OS, T = a.ga.tileSize, T2 = T * T;
+  const int P = privatePitch(T), TP = T * P;  // private tile: T rows of pitch P
   float* kt = smem;                            // [OS][K]
-  float* tile = smem + ((OS * K + 3) & ~3);    // [NIMG*2][T*T]
+  float* priv = smem + ((OS * K + 3) & ~3);    // [wave][NIMG][re, im][T][P]
+  // staged records of each wave: position key, kernel row offsets, values
+  int2* stPos = reinterpret_cast<int2*>(priv + kGridWaves * NIMG * 2 * TP);  // [wave][64]
+  float2* stVis = reinterpret_cast<float2*>(stPos + kGridWaves * 64);        // [wave][64][NIMG]
   loadKernelTable(kt, ktab, khalf, OS * K);
   const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nwaves = blockDim.x >> 6;
+  float* myTile = priv + wave * NIMG * 2 * TP;
+  int2* myPos = stPos + wave * 64;
+  float2* myVis = stVis + wave * 64 * NIMG;
   const int taps = K * K;
   int tj[TG], ti[TG];
   bool tapOk[TG];
@@ -526,143 +563,171 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
     ti[q] = tapOk[q] ? t - tj[q] * K : 0;
   }
   const size_t HW = size_t(a.ga.W) * a.ga.H;
-  const uint32_t nItems = passItems(binLo, binHi, extraStart);
+  const uint32_t nItems = passItems(binLo, binHi, itemStart);
   __syncthreads();
   for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
-    if (threadIdx.x == 0) sItem = decodeItem(item, binLo, binHi, binStart, extraStart, a.chunk);
-    __syncthreads();
-    const WorkItem wi = sItem;
+    // every lane decodes the item (uniform loads)
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk);
     const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
+    if (ACCUM && count == 0) continue;  // block-uniform; nothing to add
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
-    for (int i = threadIdx.x; i < NIMG * 2 * T2; i += blockDim.x) tile[i] = 0.0f;
-    __syncthreads();  // also: every thread has read sItem
-    if (ACCUM && count == 0) continue;  // block-uniform; nothing to add
-    for (uint32_t b0 = uint32_t(wave) * 64u; b0 < count; b0 += uint32_t(nwaves) * 64u) {
-      const uint32_t it = b0 + uint32_t(lane);
-      int key = 0, kxo = 0, kyo = 0;
-      float vx[NIMG], vy[NIMG];
-#pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi) vx[gi] = vy[gi] = 0.0f;
-      if (it < count) {
-        const uint2 rec = recs[start + it];
-        const uint32_t idx = rec.x & 0x3FFFFFFFu;
-        const bool conj = (rec.x >> 31) != 0;
-        key = int(rec.y & 4095u);  // (xo + K - 1) | (yo + K - 1) << 6
-        const int kk = int(rec.y >> 12);
-        const int ky = kk / OS;
-        kyo = ky * K;
-        kxo = (kk - ky * OS) * K;
-#pragma unroll
-        for (int gi = 0; gi < NIMG; ++gi) {
-          float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
-          if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, -1.0);
-          vx[gi] = v.x;
-          vy[gi] = conj ? -v.y : v.y;
-        }
-      }
-      const int nb = min(64, int(count - b0));
+    if (count > 0) {
+      // the wave's own copy: no other wave reads or writes it until the sum
+      for (int i = lane; i < NIMG * 2 * TP; i += 64) myTile[i] = 0.0f;
+      waveSync();
+      // wave segment of the item's records
+      const uint32_t seg = (count + kGridWaves - 1) / kGridWaves;
+      const uint32_t w0 = min(count, uint32_t(wave) * seg), w1 = min(count, w0 + seg);
       float acc[TG][2 * NIMG];
 #pragma unroll
       for (int q = 0; q < TG; ++q)
 #pragma unroll
         for (int e = 0; e < 2 * NIMG; ++e) acc[q][e] = 0.0f;
-      int cur = readLaneI(key, 0);
-      for (int s = 0; s < nb; ++s) {
-        const int k = readLaneI(key, s);
-        if (k != cur) {
-          const int xo = (cur & 63) - (K - 1), yo = (cur >> 6) - (K - 1);
-#pragma unroll
-          for (int q = 0; q < TG; ++q) {
-            const int cx = xo + ti[q], cy = yo + tj[q];
-            if (tapOk[q] && cx >= 0 && cx < T && cy >= 0 && cy < T) {
-              const int cell = cy * T + cx;
-#pragma unroll
-              for (int gi = 0; gi < NIMG; ++gi) {
-                atomicAdd(&tile[(2 * gi) * T2 + cell], acc[q][2 * gi]);
-                atomicAdd(&tile[(2 * gi + 1) * T2 + cell], acc[q][2 * gi + 1]);
-              }
-            }
-#pragma unroll
-            for (int e = 0; e < 2 * NIMG; ++e) acc[q][e] = 0.0f;
-          }
-          cur = k;
-        }
-        const int kxs = readLaneI(kxo, s), kys = readLaneI(kyo, s);
-        float sx[NIMG], sy[NIMG];
-#pragma unroll
-        for (int gi = 0; gi < NIMG; ++gi) {
-          sx[gi] = readLaneF(vx[gi], s);
-          sy[gi] = readLaneF(vy[gi], s);
-        }
-#pragma unroll
-        for (int q = 0; q < TG; ++q) {
-          // yKernelValue * xKernel[i] (gridding/wstackinggridder.cpp:574)
-          const float kv = kt[kys + tj[q]] * kt[kxs + ti[q]];
+      int cur = -1;
+      for (uint32_t b0 = w0; b0 < w1; b0 += 64u) {
+        // stage up to 64 records: lane l loads record b0 + l
+        const uint32_t it = b0 + uint32_t(lane);
+        if (it < w1) {
+          const uint2 rec = recs[start + it];
+          const uint32_t idx = rec.x & 0x3FFFFFFFu;
+          const bool conj = (rec.x >> 31) != 0;
+          const int kk = int(rec.y >> 12);
+          const int ky = kk / OS;
+          myPos[lane] = make_int2(int(rec.y & 4095u), (ky * K) | ((kk - ky * OS) * K) << 16);
 #pragma unroll
           for (int gi = 0; gi < NIMG; ++gi) {
-            acc[q][2 * gi] += sx[gi] * kv;
-            acc[q][2 * gi + 1] += sy[gi] * kv;
+            float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+            if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, -1.0);
+            if (conj) v.y = -v.y;
+            myVis[lane * NIMG + gi] = v;
           }
         }
+        waveSync();
+        const int nb = int(min(64u, w1 - b0));
+        // eight records at a time: their staged data and kernel values are
+        // read from LDS together, so one LDS latency covers eight records
+        for (int s0 = 0; s0 < nb; s0 += 8) {
+          int key[8];
+          float kv[8][TG];
+          float2 v[8][NIMG];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int su = min(s0 + u, nb - 1);
+            const int2 ps = myPos[su];  // same address in every lane: broadcast
+            key[u] = (s0 + u < nb) ? ps.x : -2;
+            const int kyo = ps.y & 0xFFFF, kxo = ps.y >> 16;
+#pragma unroll
+            for (int q = 0; q < TG; ++q)
+              // yKernelValue * xKernel[i] (gridding/wstackinggridder.cpp:574)
+              kv[u][q] = kt[kyo + tj[q]] * kt[kxo + ti[q]];
+#pragma unroll
+            for (int gi = 0; gi < NIMG; ++gi) v[u][gi] = myVis[su * NIMG + gi];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (key[u] == -2) break;  // past the batch end (wave-uniform)
+            if (key[u] != cur) {
+              if (cur >= 0) {
+                const int xo = (cur & 63) - (K - 1), yo = (cur >> 6) - (K - 1);
+#pragma unroll
+                for (int q = 0; q < TG; ++q) {
+                  const int cx = xo + ti[q], cy = yo + tj[q];
+                  if (tapOk[q] && cx >= 0 && cx < T && cy >= 0 && cy < T) {
+                    float* cell = myTile + cy * P + cx;
+                    float old[2 * NIMG];  // all loads first: one LDS latency
+#pragma unroll
+                    for (int e = 0; e < 2 * NIMG; ++e) old[e] = cell[e * TP];
+#pragma unroll
+                    for (int e = 0; e < 2 * NIMG; ++e) cell[e * TP] = old[e] + acc[q][e];
+                  }
+#pragma unroll
+                  for (int e = 0; e < 2 * NIMG; ++e) acc[q][e] = 0.0f;
+                }
+              }
+              cur = key[u];
+            }
+#pragma unroll
+            for (int q = 0; q < TG; ++q)
+#pragma unroll
+              for (int gi = 0; gi < NIMG; ++gi) {
+                acc[q][2 * gi] += v[u][gi].x * kv[u][q];
+                acc[q][2 * gi + 1] += v[u][gi].y * kv[u][q];
+              }
+          }
+        }
+        waveSync();  // staging is rewritten next batch
       }
-      {
+      if (cur >= 0) {
         const int xo = (cur & 63) - (K - 1), yo = (cur >> 6) - (K - 1);
 #pragma unroll
         for (int q = 0; q < TG; ++q) {
           const int cx = xo + ti[q], cy = yo + tj[q];
           if (tapOk[q] && cx >= 0 && cx < T && cy >= 0 && cy < T) {
-            const int cell = cy * T + cx;
+            float* cell = myTile + cy * P + cx;
+            float old[2 * NIMG];
 #pragma unroll
-            for (int gi = 0; gi < NIMG; ++gi) {
-              atomicAdd(&tile[(2 * gi) * T2 + cell], acc[q][2 * gi]);
-              atomicAdd(&tile[(2 * gi + 1) * T2 + cell], acc[q][2 * gi + 1]);
-            }
+            for (int e = 0; e < 2 * NIMG; ++e) old[e] = cell[e * TP];
+#pragma unroll
+            for (int e = 0; e < 2 * NIMG; ++e) cell[e * TP] = old[e] + acc[q][e];
           }
         }
       }
     }
-    __syncthreads();
+    __syncthreads();  // every wave's copy is complete
+    // sum of the copies, cell c of image gi (0 for an empty bin)
+    auto tileValue = [&](int gi, int cx, int cy) {
+      float2 v = make_float2(0.0f, 0.0f);
+      if (count > 0) {
+        const int o = cy * P + cx;
+#pragma unroll
+        for (int w = 0; w < kGridWaves; ++w) {
+          const float* t = priv + w * NIMG * 2 * TP + (2 * gi) * TP;
+          v.x += t[o];
+          v.y += t[TP + o];
+        }
+      }
+      return v;
+    };
     if (wi.j > 0) {
       // an extra item of a large bin: its tile goes to the partial slot
       float2* dst = partials + size_t(extraStart[bin] + wi.j - 1u) * size_t(NIMG * T2);
-      for (int c = threadIdx.x; c < T2; c += blockDim.x)
+      for (int c = threadIdx.x; c < T2; c += blockDim.x) {
+        const int cy = c / T, cx = c - cy * T;
 #pragma unroll
-        for (int gi = 0; gi < NIMG; ++gi)
-          dst[gi * T2 + c] = make_float2(tile[(2 * gi) * T2 + c], tile[(2 * gi + 1) * T2 + c]);
-      __syncthreads();
-      continue;
-    }
-    // write the tile in layer storage order (blocked: 4 rows x T columns
-    // contiguous per row block)
-    const int x0 = tx * T, y0 = ty * T;
-    for (int c = threadIdx.x; c < T2; c += blockDim.x) {
-      int cx, cy;
-      if (a.ga.blocked) {
-        const int rb = c / (4 * T), r4 = c - rb * 4 * T;
-        cx = r4 >> 2;
-        cy = rb * 4 + (r4 & 3);
-      } else {
-        cy = c / T;
-        cx = c - cy * T;
+        for (int gi = 0; gi < NIMG; ++gi) dst[gi * T2 + c] = tileValue(gi, cx, cy);
       }
-      const size_t off = cellIndex(x0 + cx, y0 + cy, a.ga.W, a.ga.blocked);
-      const int cell = cy * T + cx;
-#pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi) {
-        float2* dst = layers +
-                      (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
-        float2 v = make_float2(tile[(2 * gi) * T2 + cell], tile[(2 * gi + 1) * T2 + cell]);
-        if (ACCUM) {
-          const float2 o = *dst;
-          v.x += o.x;
-          v.y += o.y;
+    } else {
+      // write the tile in layer storage order (blocked: 4 rows x T columns
+      // contiguous per row block)
+      const int x0 = tx * T, y0 = ty * T;
+      for (int c = threadIdx.x; c < T2; c += blockDim.x) {
+        int cx, cy;
+        if (a.ga.blocked) {
+          const int rb = c / (4 * T), r4 = c - rb * 4 * T;
+          cx = r4 >> 2;
+          cy = rb * 4 + (r4 & 3);
+        } else {
+          cy = c / T;
+          cx = c - cy * T;
         }
-        *dst = v;
+        const size_t off = cellIndex(x0 + cx, y0 + cy, a.ga.W, a.ga.blocked);
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          float2* dst = layers +
+                        (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
+          float2 v = tileValue(gi, cx, cy);
+          if (ACCUM) {
+            const float2 o = *dst;
+            v.x += o.x;
+            v.y += o.y;
+          }
+          *dst = v;
+        }
       }
     }
-    __syncthreads();
+    __syncthreads();  // the copies are read before the next item clears them
   }
 }
 
@@ -674,7 +739,7 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
 degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
-                  const uint32_t* __restrict__ extraStart,
+                  const uint32_t* __restrict__ itemStart,
                   const uint2* __restrict__ recs, const float* __restrict__ ktab,
                   const __half2* __restrict__ khalf, const double* __restrict__ uvw,
                   const double* __restrict__ wl, const float2* __restrict__ layers,
@@ -687,10 +752,10 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const size_t HW = size_t(a.ga.W) * a.ga.H;
-  const uint32_t nItems = passItems(binLo, binHi, extraStart);
+  const uint32_t nItems = passItems(binLo, binHi, itemStart);
   for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
     // every lane decodes the item (uniform loads)
-    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, extraStart, a.chunk);
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk);
     const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
     if (count == 0) continue;  // block-uniform
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
@@ -746,8 +811,10 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
 
 
 // Adds the partial tiles of the extra items of every split bin of a pass to
-// the layer, in item order. One workgroup per split bin: the workgroup that
-// meets the bin's first extra item does the bin.
+// the layer, in item order (a fixed summation order). Task (e, group): if e
+// is the first extra item of its bin, the workgroup sums that bin's partial
+// tiles for one group of blockDim cells, one cell per thread (coalesced over
+// the cells of each partial tile).
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
 reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
@@ -756,31 +823,49 @@ reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const uint32_t eLo = extraStart[binLo], eHi = extraStart[binHi];
   const int T = a.ga.tileSize, T2 = T * T;
+  const uint32_t groups = uint32_t((T2 + blockDim.x - 1) / blockDim.x);
   const size_t HW = size_t(a.ga.W) * a.ga.H;
-  for (uint32_t e = eLo + blockIdx.x; e < eHi; e += gridDim.x) {
-    const uint32_t bin = binOfExtra(extraStart, binLo, binHi, e);
+  const size_t tasks = size_t(eHi - eLo) * groups;
+  for (size_t task = blockIdx.x; task < tasks; task += gridDim.x) {
+    const uint32_t e = eLo + uint32_t(task / groups);
+    const uint32_t grp = uint32_t(task - size_t(e - eLo) * groups);
+    const uint32_t bin = ownerOf(extraStart, binLo, binHi, e);
     const uint32_t first = extraStart[bin];
-    if (e != first) continue;  // block-uniform
+    if (e != first) continue;  // another task of the bin does the work
+    const int c = int(grp * blockDim.x + threadIdx.x);
+    if (c >= T2) continue;
     const uint32_t nEx = extraStart[bin + 1] - first;
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
-    for (int c = threadIdx.x; c < T2; c += blockDim.x) {
-      const int cy = c / T, cx = c - cy * T;
-      const size_t off = cellIndex(tx * T + cx, ty * T + cy, a.ga.W, a.ga.blocked);
+    const int cy = c / T, cx = c - cy * T;
+    const size_t off = cellIndex(tx * T + cx, ty * T + cy, a.ga.W, a.ga.blocked);
 #pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi) {
-        float2 sum = make_float2(0.0f, 0.0f);
-        for (uint32_t q = 0; q < nEx; ++q) {
-          const float2 v = partials[(size_t(first + q) * NIMG + gi) * T2 + c];
-          sum.x += v.x;
-          sum.y += v.y;
+    for (int gi = 0; gi < NIMG; ++gi) {
+      const float2* src = partials + (size_t(first) * NIMG + gi) * T2 + c;
+      const size_t stride = size_t(NIMG) * T2;
+      // four independent partial sums keep four loads in flight
+      float2 s4[4] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
+      uint32_t q = 0;
+      for (; q + 4 <= nEx; q += 4) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float2 v = src[size_t(q + k) * stride];
+          s4[k].x += v.x;
+          s4[k].y += v.y;
         }
-        float2* dst = layers +
-                      (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
-        const float2 o = *dst;
-        *dst = make_float2(o.x + sum.x, o.y + sum.y);
       }
+      for (; q < nEx; ++q) {
+        const float2 v = src[size_t(q) * stride];
+        s4[0].x += v.x;
+        s4[0].y += v.y;
+      }
+      const float sx = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
+      const float sy = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
+      float2* dst = layers +
+                    (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
+      const float2 o = *dst;
+      *dst = make_float2(o.x + sx, o.y + sy);
     }
   }
 }
@@ -1270,7 +1355,20 @@ size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
     const int sg = T + g->K - 1;
     return kt + size_t(nimg) * sg * sg * sizeof(float2);
   }
-  return kt + size_t(nimg) * T * T * 2 * sizeof(float);
+  // per wave: a private T x pitch tile per image component + 64 staged records
+  const size_t perWave = size_t(nimg) * 2 * T * privatePitch(T) * sizeof(float) +
+                         64 * (sizeof(int2) + size_t(nimg) * sizeof(float2));
+  return kt + kGridWaves * perWave;
+}
+
+// LDS a workgroup may use (gfx950: 160 KiB per CU).
+size_t maxLdsBytes(const hws_gridder* g) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, g->cfg.device) !=
+          hipSuccess ||
+      v <= 0)
+    v = 64 * 1024;
+  return size_t(v);
 }
 
 int persistentBlocks(const hws_gridder* g, int perCu) {
@@ -1279,14 +1377,15 @@ int persistentBlocks(const hws_gridder* g, int perCu) {
   return std::max(1, cus) * perCu;
 }
 
-// Records per work item: about a quarter of an even share of the records
-// (about 1.2 per sample) over the grid kernel's workgroups, a power of two
-// in [1024, 65536]; or the configured value.
+// Records per work item: about a sixteenth of an even share of the records
+// (about 1.2 per sample) over 4 workgroups per CU, a power of two in
+// [1024, 8192] (measured best: 4096-8192 at the 4096^2 benchmark); or the
+// configured value.
 uint32_t workChunk(const hws_gridder* g, size_t nSamples) {
   if (g->cfg.work_chunk) return uint32_t(g->cfg.work_chunk);
-  const size_t target = (nSamples + nSamples / 4) / (size_t(persistentBlocks(g, 4)) * 4) + 1;
+  const size_t target = (nSamples + nSamples / 4) / (size_t(persistentBlocks(g, 4)) * 16) + 1;
   size_t c = 1024;
-  while (c < target && c < 65536) c <<= 1;
+  while (c < target && c < 8192) c <<= 1;
   return uint32_t(c);
 }
 
@@ -1340,8 +1439,10 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     HWS_TRY(ensureAlloc(g, kp.counts, kp.countsCapacity, pl.nbins + 1, "plan counts"));
     HWS_TRY(ensureAlloc(g, kp.binStart, kp.binCapacity, pl.nbins + 1, "plan bin offsets"));
     HWS_TRY(ensureAlloc(g, kp.extraStart, kp.extraCapacity, pl.nbins + 1, "plan item offsets"));
+    HWS_TRY(ensureAlloc(g, kp.itemStart, kp.itemCapacity, pl.nbins + 1, "plan item offsets"));
     HWS_TRY(launchFill(g, kp.counts, 0u, pl.nbins + 1));
-    KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind};
+    KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind,
+                nullptr};
     if (n > 0) {
       hipLaunchKernelGGL(binKernel<false>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
                          dUvw, dWl, nrow, int(nchan), ga, ka);
@@ -1354,16 +1455,23 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                        pl.counters + 6 + kind);
     HWS_TRY(checkHip(g, hipGetLastError(), "work item launch"));
     HWS_TRY(exclusiveScan(g, kp.counts, kp.extraStart, pl.nbins, pl.scanScratch));
-    // the record and extra-item totals size the sorted array and partials
-    uint32_t totals[2] = {0, 0};
+    hipLaunchKernelGGL(binItemsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
+                       kp.binStart, kp.extraStart, pl.nbins, kp.counts);
+    HWS_TRY(checkHip(g, hipGetLastError(), "work item launch"));
+    HWS_TRY(exclusiveScan(g, kp.counts, kp.itemStart, pl.nbins, pl.scanScratch));
+    // the record and item totals size the sorted array and partials
+    uint32_t totals[3] = {0, 0, 0};
     HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[0], kp.binStart + pl.nbins, sizeof(uint32_t),
                                        hipMemcpyDeviceToHost, g->stream), "record total"));
     HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[1], kp.extraStart + pl.nbins, sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, g->stream), "item total"));
+    HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[2], kp.itemStart + pl.nbins, sizeof(uint32_t),
                                        hipMemcpyDeviceToHost, g->stream), "item total"));
     HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "plan totals"));
     const uint32_t total = totals[0];
     kp.nItems = total;
     kp.nExtra = totals[1];
+    kp.nWorkItems = totals[2];
     HWS_TRY(ensureAlloc(g, kp.recSorted, kp.capacity, size_t(total), "plan sorted records"));
     if (kp.nExtra)
       HWS_TRY(ensureAlloc(g, kp.partials, kp.partialCapacity,
@@ -1374,10 +1482,28 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                                        hipMemcpyDeviceToDevice, g->stream), "cursor copy"));
     ka.cursor = kp.counts;
     ka.recSorted = kp.recSorted;
+    // sort keys bin << 12 | origin need 12 + log2(nbins) <= 32 bits
+    int binBits = 0;
+    while ((size_t(1) << binBits) < pl.nbins) ++binBits;
+    const bool sortByOrigin = binBits <= 20 && total > 1;
+    if (sortByOrigin) {
+      HWS_TRY(ensureAlloc(g, kp.keys, kp.keyCapacity, size_t(total), "plan sort keys"));
+      HWS_TRY(ensureAlloc(g, kp.keysAlt, kp.keyAltCapacity, size_t(total), "plan sort keys"));
+      HWS_TRY(ensureAlloc(g, kp.recAlt, kp.altCapacity, size_t(total), "plan sorted records"));
+      ka.sortKeys = kp.keys;
+    }
     if (n > 0) {
       hipLaunchKernelGGL(binKernel<true>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
                          dUvw, dWl, nrow, int(nchan), ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin scatter launch"));
+    }
+    if (sortByOrigin && n > 0) {
+      uint32_t* k0 = kp.keys;
+      uint2* r0 = kp.recSorted;
+      HWS_TRY(sortRecords(g, kp.keys, kp.keysAlt, kp.recSorted, kp.recAlt, size_t(total),
+                          12 + binBits));
+      if (kp.keys != k0) std::swap(kp.keyCapacity, kp.keyAltCapacity);
+      if (kp.recSorted != r0) std::swap(kp.capacity, kp.altCapacity);
     }
   }
   pl.valid = true;
@@ -1406,21 +1532,46 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
   Plan& pl = g->plan;
   if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
   GridArgs a = gridArgs(g, pass);
-  const int blocks = persistentBlocks(g, 4);
+  const int blocks = persistentBlocks(g, 8);
   const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
   const int tg = (g->K * g->K + 63) / 64;
+  const float2* vis = reinterpret_cast<const float2*>(dVis);
+  float2* layers = reinterpret_cast<float2*>(g->dLayers);
+  const size_t ldsMax = maxLdsBytes(g);
   for (int kind = 0; kind < 2; ++kind) {
     const KindPlan& kp = pl.kinds[kind];
     if (!kp.active) continue;
-    a.imgs.n = kp.nImg;
-    for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = kp.img[i];
-    const size_t lds = gridLdsBytes(g, kp.nImg, false);
-    const float2* vis = reinterpret_cast<const float2*>(dVis);
-#define HWS_GRID_LAUNCH(N, T, ACC)                                                           \
-  hipLaunchKernelGGL((gridTilesKernel<N, T, ACC>), dim3(blocks), dim3(kGridThreads), lds,    \
-                     g->stream, a, kp.binStart, kp.extraStart, kp.recSorted, vis, g->dKernel, \
-                     khalf, pl.uvw, pl.wavelengths, reinterpret_cast<float2*>(g->dLayers),     \
-                     kp.partials)
+    // images per launch: as many as the LDS tile allows
+    int per = kp.nImg;
+    while (per > 1 && gridLdsBytes(g, per, false) > ldsMax) --per;
+    if (gridLdsBytes(g, per, false) > ldsMax)
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "uv tile and kernel table exceed LDS");
+    for (int i0 = 0; i0 < kp.nImg; i0 += per) {
+      const int n = std::min(per, kp.nImg - i0);
+      a.imgs.n = n;
+      for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = (i0 + i < kp.nImg) ? kp.img[i0 + i] : 0;
+      const size_t lds = gridLdsBytes(g, n, false);
+      if (!accumulate) {
+        // the kernel writes the tiles of non-empty bins only: clear the pass's
+        // layers of these images first
+        const size_t HW = size_t(g->W) * g->H;
+        const size_t words = size_t(a.layerHi - a.layerLo) * HW * sizeof(float2) / 16;
+        for (int i = 0; i < n; ++i) {
+          uint4* base = reinterpret_cast<uint4*>(
+              layers + (size_t(a.imgs.idx[i]) * a.nLayersPass + (a.layerLo - a.passStart)) * HW);
+          const unsigned zb = unsigned(std::min<size_t>((words + kThreads - 1) / kThreads,
+                                                        size_t(persistentBlocks(g, 16))));
+          hipLaunchKernelGGL(zero16Kernel, dim3(std::max(1u, zb)), dim3(kThreads), 0, g->stream,
+                             base, words);
+        }
+        HWS_TRY(checkHip(g, hipGetLastError(), "layer clear launch"));
+      }
+      // partial tiles of a launch hold its own images only
+#define HWS_GRID_LAUNCH(N, TG, ACC)                                                           \
+  hipLaunchKernelGGL((gridTilesKernel<N, TG, ACC>), dim3(blocks), dim3(kGridThreads), lds,      \
+                     g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.recSorted, vis, \
+                     g->dKernel,                                                                \
+                     khalf, pl.uvw, pl.wavelengths, layers, kp.partials)
 #define HWS_GRID_TG(N, ACC)          \
   if (tg == 1)                       \
     HWS_GRID_LAUNCH(N, 1, ACC);      \
@@ -1438,32 +1589,33 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
       HWS_GRID_TG(N, false)          \
     }                                \
     break;
-    switch (kp.nImg) {
-      HWS_GRID_CASE(1)
-      HWS_GRID_CASE(2)
-      HWS_GRID_CASE(3)
-      HWS_GRID_CASE(4)
-    }
+      switch (n) {
+        HWS_GRID_CASE(1)
+        HWS_GRID_CASE(2)
+        HWS_GRID_CASE(3)
+        HWS_GRID_CASE(4)
+      }
 #undef HWS_GRID_CASE
 #undef HWS_GRID_TG
 #undef HWS_GRID_LAUNCH
-    HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
-    if (kp.nExtra) {
-      const unsigned rblocks = unsigned(std::min<uint64_t>(kp.nExtra, 4096));
-      float2* layers = reinterpret_cast<float2*>(g->dLayers);
-      switch (kp.nImg) {
+      HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
+      if (kp.nExtra) {
+        const uint64_t groups = (uint64_t(pl.tileSize) * pl.tileSize + kThreads - 1) / kThreads;
+        const unsigned rblocks = unsigned(std::min<uint64_t>(kp.nExtra * groups, 65536));
+        switch (n) {
 #define HWS_REDUCE_CASE(N)                                                                    \
   case N:                                                                                     \
     hipLaunchKernelGGL(reducePartialsKernel<N>, dim3(rblocks), dim3(kThreads), 0, g->stream,   \
                        a, kp.extraStart, kp.partials, layers);                                \
     break;
-        HWS_REDUCE_CASE(1)
-        HWS_REDUCE_CASE(2)
-        HWS_REDUCE_CASE(3)
-        HWS_REDUCE_CASE(4)
+          HWS_REDUCE_CASE(1)
+          HWS_REDUCE_CASE(2)
+          HWS_REDUCE_CASE(3)
+          HWS_REDUCE_CASE(4)
 #undef HWS_REDUCE_CASE
+        }
+        HWS_TRY(checkHip(g, hipGetLastError(), "partial reduce launch"));
       }
-      HWS_TRY(checkHip(g, hipGetLastError(), "partial reduce launch"));
     }
   }
   return HWS_OK;
@@ -1486,7 +1638,7 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
 #define HWS_DEGRID_CASE(N)                                                                 \
   case N:                                                                                  \
     hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream,  \
-                       a, kp.binStart, kp.extraStart, kp.recSorted, g->dKernel, khalf, pl.uvw, \
+                       a, kp.binStart, kp.itemStart, kp.recSorted, g->dKernel, khalf, pl.uvw, \
                        pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out);  \
     break;
       HWS_DEGRID_CASE(1)
