@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-accuracy", action="store_true",
                    help="skip the error report against the FP64 direct transform")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                   help="first copy BYTES device-to-device (the known-size copy against which "
+                        "hardware byte counters of a profile are scaled)")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -171,6 +174,21 @@ def cpu_baseline(scale_rows=64, threads=None):
     }
 
 
+def fft_traffic_from_profile():
+    """HBM bytes per 4096^2 layer transform (row + column launch) measured
+    with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the same
+    counters on a copy of known size; read from the committed summary
+    (profiles/fft_blocked_4096_traffic.md), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "fft_blocked_4096_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_layer_transform:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -180,6 +198,12 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.perf_counter()
     w = Workload(args.scale, dev)
     log(f"workload ready in {time.perf_counter() - t0:.1f} s: {w.nrow} rows x {w.nchan} ch x 4 "
@@ -204,11 +228,12 @@ def main():
     dominant = max(("fft", ph["fft_inv"] + ph["fft_pred"]), ("grid", ph["grid"] + ph["degrid"]),
                    ("image", ph["project"] + ph["img2layer"]), ("bin", ph["bin"]),
                    key=lambda x: x[1])[0]
+    traffic = fft_traffic_from_profile()
     roofline = {
         "bound": "hbm",
         "kernel": "layer 2-D FFT, blocked layout (fftRowsBlockedKernel + fftColsBlockedKernel)",
         "achieved": round(fft_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(fft_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(fft_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
         "algorithmic_bytes_per_layer_transform": bytes_per_fft,
         "layer_transforms_per_direction": n_fft_2d,
         "avg_ms_per_layer_transform": round(fft_ms / n_fft_2d, 5) if n_fft_2d else None,
