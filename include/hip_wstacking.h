@@ -221,6 +221,28 @@ hws_status hws_get_stats(const hws_gridder* g, hws_stats* out);
 /* Wait for all work of the context. */
 hws_status hws_synchronize(hws_gridder* g);
 
+/* ---- graph capture of the device-resident forms (no reference
+ * counterpart) ----
+ * hws_graph_capture_invert / _predict capture the whole
+ * hws_invert_planned_device / hws_predict_planned_device sequence (clear,
+ * grid or image -> layers, transforms, corrections, degrid) of a planned
+ * context in a batched precision with its layers in one pass into an
+ * executable HIP graph, so a
+ * major cycle replays it with one launch. The graph keeps the pointers given
+ * here and the context's plan buffers: it is valid until the context is
+ * planned again, re-prepared or destroyed. Phase timings (hws_get_stats) are
+ * not recorded by graph launches. Captures run on the context's own stream;
+ * hws_graph_launch enqueues a replay on hip_stream (NULL: the null stream). */
+typedef struct hws_graph hws_graph;
+hws_status hws_graph_capture_invert(hws_gridder* g, const float* d_vis,
+                                    double multiplication_factor, float* d_real,
+                                    float* d_imag, hws_graph** out);
+hws_status hws_graph_capture_predict(hws_gridder* g, const float* d_real,
+                                     const float* d_imag, float* d_vis_out,
+                                     hws_graph** out);
+hws_status hws_graph_launch(hws_graph* graph, void* hip_stream);
+void hws_graph_destroy(hws_graph* graph);
+
 /* ---- FP64 direct Fourier transform (no context) ----
  * Pixel (x, y) of the width x height image has l = (W/2 - x) dl + l_shift,
  * m = (y - H/2) dm + m_shift, n' = sqrt(1 - l^2 - m^2) - 1 (0 outside the

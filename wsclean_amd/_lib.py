@@ -86,6 +86,10 @@ SIGNATURES = {
     "hws_predict_device": (_i, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp]),
     "hws_get_stats": (_i, [_vp, C.POINTER(HwsStats)]),
     "hws_synchronize": (_i, [_vp]),
+    "hws_graph_capture_invert": (_i, [_vp, _vp, _d, _vp, _vp, C.POINTER(_vp)]),
+    "hws_graph_capture_predict": (_i, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "hws_graph_launch": (_i, [_vp, _vp]),
+    "hws_graph_destroy": (None, [_vp]),
     "hws_direct_invert_device": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _sz, _sz, _vp, _vp,
                                       _vp, _sz, _d, _vp]),
     "hws_direct_predict_device": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _vp, _sz, _sz, _vp,

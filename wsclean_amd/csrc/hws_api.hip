@@ -217,6 +217,7 @@ hws_status stageRows(hws_gridder* g, size_t nrow, size_t nchan,
 }
 
 hws_status record(hws_gridder* g, int i) {
+  if (g->capturing) return HWS_OK;  // phase timings are not part of a captured graph
   return checkHip(g, hipEventRecord(g->ev[i], g->stream), "event record");
 }
 
@@ -857,6 +858,105 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
   }
   *out = s;
   return HWS_OK;
+}
+
+// ---- graph capture of the device-resident forms
+struct hws_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+};
+
+}  // extern "C"
+
+namespace {
+
+hws_status graphChecks(hws_gridder* g, hws_graph** out) {
+  if (!g || !out) return fail(g, HWS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (!g->plan.valid) return fail(g, HWS_ERR_BAD_STATE, "hws_plan_device has not been called");
+  if (g->nPasses != 1)
+    return fail(g, HWS_ERR_UNSUPPORTED, "graph capture needs the layers in a single pass");
+  if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32)
+    return fail(g, HWS_ERR_UNSUPPORTED, "graph capture is for the batched precisions");
+  if (g->capturing) return fail(g, HWS_ERR_BAD_STATE, "a capture is already in progress");
+  return HWS_OK;
+}
+
+// Runs op on the context's own stream (the null stream cannot be captured)
+// while capturing it, then instantiates the graph.
+template <class Op>
+hws_status captureGraph(hws_gridder* g, Op op, hws_graph** out) {
+  (void)hipSetDevice(g->cfg.device);
+  hipStream_t saved = g->stream;
+  g->stream = g->ownStream;
+  HWS_TRY(checkHip(g, hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal),
+                   "begin capture"));
+  g->capturing = true;
+  const hws_status st = op();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+  g->capturing = false;
+  g->stream = saved;
+  if (st != HWS_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  HWS_TRY(checkHip(g, e, "end capture"));
+  hipGraphExec_t exec = nullptr;
+  hws_status ist = checkHip(g, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0),
+                            "graph instantiate");
+  if (ist != HWS_OK) {
+    (void)hipGraphDestroy(graph);
+    return ist;
+  }
+  hws_graph* gr = new (std::nothrow) hws_graph;
+  if (!gr) {
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return fail(g, HWS_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
+  gr->graph = graph;
+  gr->exec = exec;
+  gr->device = g->cfg.device;
+  *out = gr;
+  return HWS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hws_status hws_graph_capture_invert(hws_gridder* g, const float* d_vis, double mult,
+                                    float* d_real, float* d_imag, hws_graph** out) {
+  HWS_TRY(graphChecks(g, out));
+  return captureGraph(
+      g, [&] { return hws_invert_planned_device(g, d_vis, mult, d_real, d_imag); }, out);
+}
+
+hws_status hws_graph_capture_predict(hws_gridder* g, const float* d_real, const float* d_imag,
+                                     float* d_vis_out, hws_graph** out) {
+  HWS_TRY(graphChecks(g, out));
+  (void)hipSetDevice(g->cfg.device);
+  HWS_TRY(ensureModel(g));  // allocations cannot be captured
+  HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "model allocation"));
+  return captureGraph(
+      g, [&] { return hws_predict_planned_device(g, d_real, d_imag, d_vis_out); }, out);
+}
+
+hws_status hws_graph_launch(hws_graph* gr, void* hip_stream) {
+  if (!gr) return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "null graph");
+  (void)hipSetDevice(gr->device);
+  return checkHip(nullptr, hipGraphLaunch(gr->exec, static_cast<hipStream_t>(hip_stream)),
+                  "graph launch");
+}
+
+void hws_graph_destroy(hws_graph* gr) {
+  if (!gr) return;
+  (void)hipSetDevice(gr->device);
+  if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+  if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  delete gr;
 }
 
 }  // extern "C"

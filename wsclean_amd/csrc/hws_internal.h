@@ -116,6 +116,7 @@ struct hws_gridder {
   bool anyReal = false, anyComplex = false;
   hipStream_t ownStream = nullptr, stream = nullptr;
   bool userStream = false;  // stream was set by hws_set_stream (may be the null stream)
+  bool capturing = false;   // a graph capture is in progress: no event records
   std::string err;
 
   // kernels

@@ -231,7 +231,48 @@ class DeviceGridder(_Context):
     def synchronize(self):
         self._check(self.L.hws_synchronize(self.h))
 
+    def capture_invert(self, vis, multiplication_factor, real_out, imag_out=None):
+        """A HIP graph of invert() on the planned samples (single pass);
+        replay with .launch(). Valid until the next plan()."""
+        gr = C.c_void_p()
+        self._check(self.L.hws_graph_capture_invert(self.h, _ptr(vis), multiplication_factor,
+                                                    _ptr(real_out), _ptr(imag_out),
+                                                    C.byref(gr)))
+        return Graph(self, gr, (vis, real_out, imag_out))
+
+    def capture_predict(self, real, imag, vis_out):
+        """A HIP graph of predict() on the planned samples (single pass)."""
+        gr = C.c_void_p()
+        self._check(self.L.hws_graph_capture_predict(self.h, _ptr(real), _ptr(imag),
+                                                     _ptr(vis_out), C.byref(gr)))
+        return Graph(self, gr, (real, imag, vis_out))
+
     def stats(self):
         s = HwsStats()
         self._check(self.L.hws_get_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in HwsStats._fields_}
+
+
+class Graph:
+    """A captured invert or predict; launch() replays it on torch's current
+    stream (or the given one)."""
+
+    def __init__(self, owner, handle, keep):
+        self.owner, self.h, self._keep = owner, handle, keep  # keep tensors alive
+
+    def launch(self, stream=None):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream(
+            torch.device("cuda", self.owner.device))
+        self.owner._check(self.owner.L.hws_graph_launch(self.h, C.c_void_p(st.cuda_stream)))
+
+    def close(self):
+        if self.h:
+            self.owner.L.hws_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
