@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--scale", type=float, default=1.0,
                    help="fraction of the 1800 timesteps (1.0 = full workload)")
     p.add_argument("--tile", type=int, default=0)
+    p.add_argument("--chunk", type=int, default=0, help="records per grid work item (0: auto)")
+    p.add_argument("--autotune", action="store_true",
+                   help="choose tile size and work chunk by timing candidates first")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-accuracy", action="store_true",
@@ -85,11 +88,12 @@ class Workload:
         self.imag = torch.zeros_like(self.real)
 
 
-def run_form(wl_, precision, steps, warmup, tile, device):
+def run_form(wl_, precision, steps, warmup, tile, device, chunk=0):
     import torch
     import wsclean_amd
     g = wsclean_amd.DeviceGridder(wl_.W, wl_.H, wl_.PIXEL, wl_.PIXEL, 7, 63, precision=precision,
-                                  n_images=4, complex_images=wl_.COMPLEX, tile_size=tile)
+                                  n_images=4, complex_images=wl_.COMPLEX, tile_size=tile,
+                                  work_chunk=chunk)
     npass = g.prepare_w_layers(wl_.NW, wl_.min_w, wl_.max_w)
     phases = {"bin": 0.0, "grid": 0.0, "fft_inv": 0.0, "project": 0.0,
               "img2layer": 0.0, "fft_pred": 0.0, "degrid": 0.0}
@@ -209,8 +213,18 @@ def main():
     log(f"workload ready in {time.perf_counter() - t0:.1f} s: {w.nrow} rows x {w.nchan} ch x 4 "
         f"= {w.nvis} vis, w in [{w.min_w:.1f}, {w.max_w:.1f}]")
 
+    tuned = None
+    if args.autotune:
+        from wsclean_amd.autotune import autotune
+        t = time.perf_counter()
+        tuned = autotune(w.W, w.H, w.PIXEL, w.PIXEL, w.uvw, w.wl, w.vis, n_images=4,
+                         complex_images=w.COMPLEX, precision=args.precision,
+                         min_layers=Workload.NW, chunks=(4096, 8192), repeats=1)
+        args.tile, args.chunk = tuned["tile_size"], tuned["work_chunk"]
+        log(f"autotune in {time.perf_counter() - t:.1f} s: tile {args.tile}, chunk {args.chunk}, "
+            f"suggested layers {tuned['n_w_layers']}; trials {tuned['trials']}")
     ms_step, npass, ph, st_inv, st_pred = run_form(w, args.precision, args.steps, args.warmup,
-                                                   args.tile, dev)
+                                                   args.tile, dev, args.chunk)
     log(f"{args.precision}: {ms_step * 1e3:.1f} ms/step, passes={npass}, phases(ms)={ph}")
     value = w.nvis / ms_step
 
@@ -290,6 +304,8 @@ def main():
         "rms_err_vs_fp64_dft": (accuracy.get(f"{args.precision}_os63", {}).get("image_rms_rel")
                                 if accuracy else None),
         "accuracy": accuracy,
+        "autotune": ({k: tuned[k] for k in ("tile_size", "work_chunk", "trials")}
+                     if tuned else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
