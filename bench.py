@@ -50,6 +50,11 @@ def parse():
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="first copy BYTES device-to-device (the known-size copy against which "
                         "hardware byte counters of a profile are scaled)")
+    p.add_argument("--no-facets", action="store_true", help="skip the facet-batch leg")
+    p.add_argument("--facets-only", action="store_true",
+                   help="run only the facet-batch leg and print its JSON object")
+    p.add_argument("--facet-scale", type=float, default=0.25,
+                   help="fraction of the 1800 timesteps in the facet leg")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -178,6 +183,150 @@ def cpu_baseline(scale_rows=64, threads=None):
     }
 
 
+def facet_set(rng, n=16, lo=512, hi=2048, radius_deg=5.0, pixel=None):
+    """n facets with box sizes uniform in [lo, hi] (gridded at the next
+    multiple of 64 with no prime factor above 7) and centres uniform over
+    the disc of radius_deg about the phase centre."""
+    from wsclean_amd.facets import Facet, gridded_size
+    out = []
+    for i in range(n):
+        box = int(rng.integers(lo, hi + 1))
+        size = gridded_size(box, 1.0, 64)
+        r = np.deg2rad(radius_deg) * np.sqrt(rng.uniform())
+        th = rng.uniform(0.0, 2.0 * np.pi)
+        out.append(Facet(size, size, float(np.sin(r) * np.cos(th)),
+                         float(np.sin(r) * np.sin(th)), (0, 0, box, box), f"f{i}"))
+    return out
+
+
+def facet_leg(args, dev):
+    """Facets of one major cycle: invert + predict of 16 uneven facets over
+    the same samples, (a) captured into one HIP graph per direction, (b)
+    launched facet by facet with the same kernels, (c) in the plain
+    per-channel FP32 form, facet by facet. w-layers per facet are the
+    reference's suggestion; tile size and work chunk come from the autotuner
+    on the largest facet. Plans are built once per facet
+    set (the samples do not change between major cycles) and timed apart."""
+    import torch
+    from wsclean_amd import synth
+    from wsclean_amd.autotune import autotune
+    from wsclean_amd.facets import FacetBatch
+    import wsclean_amd
+    obs = synth.config2(scale=args.facet_scale)
+    pix = Workload.PIXEL
+    nrow, nchan = obs.nrow, obs.nchan
+    nvis = nrow * nchan * 4
+    uvw = torch.from_numpy(obs.uvw_m).to(dev)
+    wl = torch.from_numpy(obs.wavelengths).to(dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4040)
+    vis = torch.randn((nrow, nchan, 4), dtype=torch.complex64, device=dev, generator=gen)
+    facets = facet_set(np.random.default_rng(404), pixel=pix)
+    big = max(facets, key=lambda f: f.width)
+    t = time.perf_counter()
+    tuned = autotune(big.width, big.height, pix, pix, uvw, wl, vis, n_images=4,
+                     complex_images=Workload.COMPLEX, l_shift=big.l_shift, m_shift=big.m_shift,
+                     chunks=(0, 8192, 16384), repeats=1)
+    log(f"facets: autotune on {big.width}^2 in {time.perf_counter() - t:.1f} s: tile "
+        f"{tuned['tile_size']} chunk {tuned['work_chunk']}")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fb = FacetBatch(facets, pix, pix, uvw, wl, n_images=4, complex_images=Workload.COMPLEX,
+                    tile_size=tuned["tile_size"], work_chunk=tuned["work_chunk"])
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t) * 1e3
+    log(f"facets: {len(facets)} planned in {plan_ms:.0f} ms; sizes "
+        f"{[f.width for f in facets]}, w-layers {fb.n_w}, passes {fb.passes}")
+    re, im = fb.images(), fb.images()
+    outs = [torch.empty_like(vis) for _ in facets]
+
+    def timed(step, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k * 1e3
+
+    def eager():
+        fb.invert(vis, 1.0, re, im)
+        fb.predict(re, im, outs)
+
+    eager()
+    eager_ms = timed(eager, args.steps)
+    ref_img = re[0].clone()
+    ginv = fb.capture_invert(vis, 1.0, re, im)
+    gpred = fb.capture_predict(re, im, outs)
+
+    def graph():
+        ginv.launch()
+        gpred.launch()
+
+    graph()
+    graph_ms = timed(graph, args.steps)
+    same = bool(torch.equal(re[0], ref_img))
+    ginv.close()
+    gpred.close()
+    n_w = list(fb.n_w)
+    fb.close()
+    del re, im, outs
+    torch.cuda.empty_cache()
+    log(f"facets: graph {graph_ms:.1f} ms/step, eager {eager_ms:.1f} ms/step")
+
+    plain = None
+    if not args.no_plain:
+        # the plain form's cost is linear in the channel count (one launch
+        # per channel) plus a fixed per-facet part: time 4 and 8 channels
+        # and extrapolate to all of them
+        def plain_ms(c):
+            v = vis[:, :c].contiguous()
+            wlc = wl[:c].contiguous()
+            total = 0.0
+            for f, nw in zip(facets, n_w):
+                g = wsclean_amd.DeviceGridder(f.width, f.height, pix, pix, 7, 63,
+                                              precision="plain", n_images=4,
+                                              complex_images=Workload.COMPLEX,
+                                              l_shift=f.l_shift, m_shift=f.m_shift)
+                g.prepare_w_layers(nw, fb.w_limits[0], fb.w_limits[1])
+                r = torch.zeros((4, f.height, f.width), dtype=torch.float32, device=dev)
+                i_ = torch.zeros_like(r)
+                o = torch.empty_like(v)
+                g.plan(uvw, wlc)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                g.invert(v, 1.0, r, i_)
+                g.predict(r, i_, o)
+                g.synchronize()
+                total += time.perf_counter() - t0
+                g.close()
+            return total * 1e3
+        try:
+            p4, p8 = plain_ms(4), plain_ms(8)
+            p_full = p4 + (p8 - p4) / 4.0 * (nchan - 4)
+            plain = {"ms_per_step": p_full, "value": nvis / (p_full * 1e-3),
+                     "ms_at_4_channels": p4, "ms_at_8_channels": p8,
+                     "method": f"plain per-channel FP32 form facet by facet, timed at 4 and 8 of "
+                               f"{nchan} channels and extrapolated linearly"}
+            log(f"facets: plain {p_full:.0f} ms/step (extrapolated from {p4:.0f}, {p8:.0f})")
+        except Exception as e:  # reported, never hidden
+            plain = {"error": str(e)}
+    del vis
+    torch.cuda.empty_cache()
+    return {
+        "workload": (f"{len(facets)} facets, boxes uniform in 512..2048 px at 4 arcsec, centres "
+                     f"uniform within 5 deg; {obs.meta['n_baselines']} baselines x "
+                     f"{obs.n_times} x 2 s x {nchan} ch x 4 corr = {nvis} visibilities; "
+                     "invert + predict per step, plans built once"),
+        "sizes": [f.width for f in facets], "w_layers": n_w,
+        "tile_size": tuned["tile_size"], "work_chunk": tuned["work_chunk"],
+        "plan_ms": plan_ms, "graph_ms_per_step": graph_ms, "eager_ms_per_step": eager_ms,
+        "graph_value": nvis / (graph_ms * 1e-3), "eager_value": nvis / (eager_ms * 1e-3),
+        "unit": "visibilities/s", "graph_equals_eager": same, "plain_form": plain,
+        "speedup_graph_vs_plain": (plain["ms_per_step"] / graph_ms
+                                   if plain and "ms_per_step" in plain else None),
+    }
+
+
 def fft_traffic_from_profile():
     """HBM bytes per 4096^2 layer transform (row + column launch) measured
     with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the same
@@ -208,6 +357,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.facets_only:
+        print(json.dumps({"facets": facet_leg(args, dev)}))
+        return
     t0 = time.perf_counter()
     w = Workload(args.scale, dev)
     log(f"workload ready in {time.perf_counter() - t0:.1f} s: {w.nrow} rows x {w.nchan} ch x 4 "
@@ -273,6 +425,17 @@ def main():
             accuracy = {"error": str(e)}
         log(f"accuracy report in {time.perf_counter() - t:.1f} s: {accuracy}")
 
+    facets = None
+    if not args.no_facets:
+        del w.vis_out
+        torch.cuda.empty_cache()
+        t = time.perf_counter()
+        try:
+            facets = facet_leg(args, dev)
+        except Exception as e:  # reported, never hidden
+            facets = {"error": f"{type(e).__name__}: {e}"}
+        log(f"facet leg in {time.perf_counter() - t:.1f} s")
+
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -306,6 +469,7 @@ def main():
         "accuracy": accuracy,
         "autotune": ({k: tuned[k] for k in ("tile_size", "work_chunk", "trials")}
                      if tuned else None),
+        "facets": facets,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

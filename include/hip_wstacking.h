@@ -240,6 +240,20 @@ hws_status hws_graph_capture_invert(hws_gridder* g, const float* d_vis,
 hws_status hws_graph_capture_predict(hws_gridder* g, const float* d_real,
                                      const float* d_imag, float* d_vis_out,
                                      hws_graph** out);
+/* The same for n contexts at once (e.g. the facets of one major cycle, each
+ * context with its own image size and phase centre): one graph that runs
+ * every context's invert (predict) in order, replayed with one launch. All
+ * contexts must be on one device, planned and single-pass; arrays hold one
+ * entry per context (d_imag may be NULL, or hold NULL entries). */
+hws_status hws_graph_capture_invert_batch(hws_gridder* const* gridders, size_t n,
+                                          const float* const* d_vis,
+                                          const double* multiplication_factors,
+                                          float* const* d_real, float* const* d_imag,
+                                          hws_graph** out);
+hws_status hws_graph_capture_predict_batch(hws_gridder* const* gridders, size_t n,
+                                           const float* const* d_real,
+                                           const float* const* d_imag,
+                                           float* const* d_vis_out, hws_graph** out);
 hws_status hws_graph_launch(hws_graph* graph, void* hip_stream);
 void hws_graph_destroy(hws_graph* graph);
 

@@ -198,6 +198,33 @@ def test_shifted_phase_centre_with_rotation(hws, oracle):
     assert rel(re[0], img_o) < TOL_IMG["reference"]
 
 
+@pytest.mark.parametrize("precision", ["reference", "mixed"])
+def test_shifted_phase_centre_predict_with_rotation(hws, oracle, precision):
+    # WSMSGridder rotates predicted samples by exp(+2 pi i (u dl + v dm)/lambda)
+    # after sampling (gridding/wsmsgridder.cpp:326-332); the sampling n'
+    # table uses the shifted l, m (gridding/wstackinggridder.cpp:909-932)
+    w = 64
+    uvw, wl, _, dl = case(7, w)
+    shift = (-0.011, 0.017)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    model = np.random.default_rng(23).standard_normal((w, w)).astype(np.float32)
+    v_o = oracle_predict(oracle, w, w, dl, dl, uvw_l, model, None, 8, 0.0, maxw, shift=shift)
+    ph = 2 * np.pi * (uvw_l[:, 0] * shift[0] + uvw_l[:, 1] * shift[1])
+    v_o = v_o * np.exp(1j * ph)
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, precision=precision, l_shift=shift[0],
+                             m_shift=shift[1], rotate_visibilities=True)
+    g.prepare_w_layers(8, 0.0, 0.0, maxw)
+    g.initialize_prediction(model)
+    g.start_prediction_pass(0)
+    v = g.sample_data(uvw, wl).reshape(-1)
+    assert np.array_equal(np.isnan(v), np.isnan(v_o))
+    ok = ~np.isnan(v_o)
+    e = rel(v[ok], v_o[ok])
+    print(f"shifted predict {precision}: rel L2 vs oracle float = {e:.2e}")
+    assert e < TOL_IMG[precision]
+
+
 def test_multipass_equals_single_pass(hws):
     w = 64
     uvw, wl, vis, dl = case(6, w)

@@ -88,6 +88,10 @@ SIGNATURES = {
     "hws_synchronize": (_i, [_vp]),
     "hws_graph_capture_invert": (_i, [_vp, _vp, _d, _vp, _vp, C.POINTER(_vp)]),
     "hws_graph_capture_predict": (_i, [_vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "hws_graph_capture_invert_batch": (_i, [C.POINTER(_vp), _sz, C.POINTER(_vp), _pd,
+                                            C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "hws_graph_capture_predict_batch": (_i, [C.POINTER(_vp), _sz, C.POINTER(_vp),
+                                             C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "hws_graph_launch": (_i, [_vp, _vp]),
     "hws_graph_destroy": (None, [_vp]),
     "hws_direct_invert_device": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _sz, _sz, _vp, _vp,

@@ -8,7 +8,8 @@ w-layer count for one set of samples on one device.
   images, minW otherwise. More layers never run faster,
   so the tuner keeps the larger of that bound and the caller's minimum, and
   reports how many passes the device memory allows.
-- tile size and work chunk: every candidate pair is timed (plan + invert +
+- tile size and work chunk (0 = the library's own rule from the sample
+  count): every candidate pair is timed (plan + invert +
   predict on the caller's samples, best of `repeats` after one warm-up, host
   clock around device-synchronised runs), and the fastest is chosen.
 """

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <new>
+#include <vector>
 
 using namespace hws;
 
@@ -865,6 +866,7 @@ struct hws_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int device = 0;
+  std::vector<hipEvent_t> events;  // fork/join events of a batch capture
 };
 
 }  // extern "C"
@@ -883,43 +885,116 @@ hws_status graphChecks(hws_gridder* g, hws_graph** out) {
   return HWS_OK;
 }
 
-// Runs op on the context's own stream (the null stream cannot be captured)
-// while capturing it, then instantiates the graph.
+// Captures op(i) for the n contexts into one graph. Context 0 runs on its
+// own stream, where the capture starts (the null stream cannot be captured);
+// every other context runs on its own stream too, forked from the start of
+// the capture and joined at its end, so the graph holds one independent
+// branch per context and the runtime may overlap them. Each context is
+// switched to its stream while it is captured.
 template <class Op>
-hws_status captureGraph(hws_gridder* g, Op op, hws_graph** out) {
+hws_status captureBatch(hws_gridder* const* gs, size_t n, Op op, hws_graph** out) {
+  hws_gridder* g = gs[0];
   (void)hipSetDevice(g->cfg.device);
-  hipStream_t saved = g->stream;
-  g->stream = g->ownStream;
-  HWS_TRY(checkHip(g, hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal),
-                   "begin capture"));
-  g->capturing = true;
-  const hws_status st = op();
+  hipStream_t cap = g->ownStream;
+  std::vector<hipStream_t> saved(n);
+  std::vector<hipEvent_t> events;
+  auto dropEvents = [&] {
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+  };
+  for (size_t i = 0; i < n; ++i) {
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      dropEvents();
+      return fail(g, HWS_ERR_HIP, "event creation failed");
+    }
+    events.push_back(ev);  // events[0] forks, events[i > 0] joins branch i
+  }
+  if (checkHip(g, hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal),
+               "begin capture") != HWS_OK) {
+    dropEvents();
+    return HWS_ERR_HIP;
+  }
+  hipError_t ce = hipEventRecord(events[0], cap);
+  for (size_t i = 0; i < n; ++i) {
+    saved[i] = gs[i]->stream;
+    gs[i]->stream = i == 0 ? cap : gs[i]->ownStream;
+    gs[i]->capturing = true;
+    if (i > 0 && ce == hipSuccess) ce = hipStreamWaitEvent(gs[i]->ownStream, events[0], 0);
+  }
+  hws_status st = ce == hipSuccess ? HWS_OK : fail(g, HWS_ERR_HIP, "capture fork failed");
+  for (size_t i = 0; i < n && st == HWS_OK; ++i) {
+    st = op(i);
+    if (st != HWS_OK && gs[i] != g) g->err = gs[i]->err;
+  }
+  for (size_t i = 1; i < n; ++i) {
+    // join every forked branch back, also after a failure, so that the
+    // capture can end
+    hipError_t je = hipEventRecord(events[i], gs[i]->ownStream);
+    if (je == hipSuccess) je = hipStreamWaitEvent(cap, events[i], 0);
+    if (je != hipSuccess && st == HWS_OK) st = fail(g, HWS_ERR_HIP, "capture join failed");
+  }
   hipGraph_t graph = nullptr;
-  const hipError_t e = hipStreamEndCapture(g->stream, &graph);
-  g->capturing = false;
-  g->stream = saved;
+  const hipError_t e = hipStreamEndCapture(cap, &graph);
+  for (size_t i = 0; i < n; ++i) {
+    gs[i]->capturing = false;
+    gs[i]->stream = saved[i];
+  }
   if (st != HWS_OK) {
     if (graph) (void)hipGraphDestroy(graph);
+    dropEvents();
     return st;
   }
-  HWS_TRY(checkHip(g, e, "end capture"));
+  if (checkHip(g, e, "end capture") != HWS_OK) {
+    dropEvents();
+    return HWS_ERR_HIP;
+  }
   hipGraphExec_t exec = nullptr;
   hws_status ist = checkHip(g, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0),
                             "graph instantiate");
   if (ist != HWS_OK) {
     (void)hipGraphDestroy(graph);
+    dropEvents();
     return ist;
   }
   hws_graph* gr = new (std::nothrow) hws_graph;
   if (!gr) {
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
+    dropEvents();
     return fail(g, HWS_ERR_OUT_OF_MEMORY, "host allocation failed");
   }
   gr->graph = graph;
   gr->exec = exec;
   gr->device = g->cfg.device;
+  gr->events = std::move(events);
   *out = gr;
+  return HWS_OK;
+}
+
+// Checks the contexts of a batch capture (each as for a single capture, all on
+// one device, none twice) and, for predict, allocates their model buffers.
+// A failure is reported on the first context as well as on the failing one.
+hws_status batchChecks(hws_gridder* const* gs, size_t n, bool predict) {
+  for (size_t i = 0; i < n; ++i) {
+    if (!gs[i]) return fail(gs[0], HWS_ERR_INVALID_ARGUMENT, "null context in the batch");
+    hws_graph* dummy = nullptr;
+    hws_status st = graphChecks(gs[i], &dummy);
+    if (st == HWS_OK && gs[i]->cfg.device != gs[0]->cfg.device)
+      st = fail(gs[i], HWS_ERR_INVALID_ARGUMENT, "contexts of a batch share one device");
+    for (size_t j = 0; j < i && st == HWS_OK; ++j)
+      if (gs[j] == gs[i])
+        st = fail(gs[i], HWS_ERR_INVALID_ARGUMENT, "a context appears twice in the batch");
+    if (st == HWS_OK && predict) {
+      (void)hipSetDevice(gs[i]->cfg.device);
+      st = ensureModel(gs[i]);  // allocations cannot be captured
+      if (st == HWS_OK)
+        st = checkHip(gs[i], hipStreamSynchronize(gs[i]->stream), "model allocation");
+    }
+    if (st != HWS_OK) {
+      if (gs[i] != gs[0]) gs[0]->err = gs[i]->err;
+      return st;
+    }
+  }
   return HWS_OK;
 }
 
@@ -930,8 +1005,9 @@ extern "C" {
 hws_status hws_graph_capture_invert(hws_gridder* g, const float* d_vis, double mult,
                                     float* d_real, float* d_imag, hws_graph** out) {
   HWS_TRY(graphChecks(g, out));
-  return captureGraph(
-      g, [&] { return hws_invert_planned_device(g, d_vis, mult, d_real, d_imag); }, out);
+  return captureBatch(
+      &g, 1, [&](size_t) { return hws_invert_planned_device(g, d_vis, mult, d_real, d_imag); },
+      out);
 }
 
 hws_status hws_graph_capture_predict(hws_gridder* g, const float* d_real, const float* d_imag,
@@ -940,8 +1016,37 @@ hws_status hws_graph_capture_predict(hws_gridder* g, const float* d_real, const 
   (void)hipSetDevice(g->cfg.device);
   HWS_TRY(ensureModel(g));  // allocations cannot be captured
   HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "model allocation"));
-  return captureGraph(
-      g, [&] { return hws_predict_planned_device(g, d_real, d_imag, d_vis_out); }, out);
+  return captureBatch(
+      &g, 1, [&](size_t) { return hws_predict_planned_device(g, d_real, d_imag, d_vis_out); },
+      out);
+}
+
+hws_status hws_graph_capture_invert_batch(hws_gridder* const* gs, size_t n,
+                                          const float* const* d_vis, const double* mult,
+                                          float* const* d_real, float* const* d_imag,
+                                          hws_graph** out) {
+  if (!gs || n == 0 || !gs[0] || !d_vis || !mult || !d_real || !out)
+    return fail(gs && n ? gs[0] : nullptr, HWS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  HWS_TRY(batchChecks(gs, n, false));
+  return captureBatch(gs, n, [&](size_t i) {
+    return hws_invert_planned_device(gs[i], d_vis[i], mult[i], d_real[i],
+                                     d_imag ? d_imag[i] : nullptr);
+  }, out);
+}
+
+hws_status hws_graph_capture_predict_batch(hws_gridder* const* gs, size_t n,
+                                           const float* const* d_real,
+                                           const float* const* d_imag,
+                                           float* const* d_vis_out, hws_graph** out) {
+  if (!gs || n == 0 || !gs[0] || !d_real || !d_vis_out || !out)
+    return fail(gs && n ? gs[0] : nullptr, HWS_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  HWS_TRY(batchChecks(gs, n, true));
+  return captureBatch(gs, n, [&](size_t i) {
+    return hws_predict_planned_device(gs[i], d_real[i], d_imag ? d_imag[i] : nullptr,
+                                      d_vis_out[i]);
+  }, out);
 }
 
 hws_status hws_graph_launch(hws_graph* gr, void* hip_stream) {
@@ -956,6 +1061,7 @@ void hws_graph_destroy(hws_graph* gr) {
   (void)hipSetDevice(gr->device);
   if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
   if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  for (hipEvent_t ev : gr->events) (void)hipEventDestroy(ev);
   delete gr;
 }
 
