@@ -97,7 +97,10 @@ typedef struct hws_config {
    * does (gridding/wsmsgridder.cpp:185-190, 328-332). 0: only the n' shift
    * of WStackingGridder::SetDenormalPhaseCentre. */
   int rotate_visibilities;
-  size_t tile_size;             /* uv tile edge for binning; 0 = default */
+  /* uv tile edge for binning; 0 = the edge closest to 16 that divides the
+   * image (16: the matrix-core grid kernel; 8: the VALU cell kernel; other
+   * edges: the private-tile kernel over origin-sorted records). */
+  size_t tile_size;
   /* Records per work item of the binned grid/degrid kernels: a (layer, tile)
    * bin with more records is split into several items (whose tiles are
    * summed in a fixed order afterwards). 0 = chosen from the sample count. */

@@ -404,3 +404,66 @@ def test_split_bins_match_oracle(hws, oracle, precision):
     ok = ~np.isnan(v_o)
     assert np.array_equal(np.isnan(v), ~ok)
     assert rel(v[ok], v_o[ok]) < TOL_IMG[precision]
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32, 24])
+def test_tile_sizes_match_oracle(hws, oracle, tile):
+    # every grid kernel form: tiles 8 and 16 take the cell-owning kernel
+    # (sums in registers, records in any order), 24 and 32 the private-tile
+    # kernel over origin-sorted records; two real images and one complex
+    # one, a shifted and rotated phase centre, bins split into work items
+    # (work_chunk 37) on a dense uv centre, through the host protocol (two
+    # batches: the accumulate path) and the device form (the tile-writing
+    # path), then predict
+    import torch
+    w = 96
+    uvw, wl, vis, dl = case(40, w, nrow=900, nchan=3, frac=0.1, nimg=3)
+    shift = (0.006, -0.004)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    nw = 6
+    ph = np.exp(1j * -2 * np.pi * (uvw_l[:, 0] * shift[0] + uvw_l[:, 1] * shift[1]))
+    rot = [(vis[:, :, k].reshape(-1).astype(np.complex128) * ph).astype(np.complex64)
+           for k in range(3)]
+    img0, _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, rot[0], nw, 0.0, maxw, shift=shift)
+    img1, _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, rot[1], nw, 0.0, maxw, shift=shift)
+    (img2r, img2i), _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, rot[2], nw, 0.0, maxw,
+                                      cplx=True, shift=shift)
+    kw = dict(precision="mixed", n_images=3, complex_images=(2,), l_shift=shift[0],
+              m_shift=shift[1], rotate_visibilities=True, tile_size=tile, work_chunk=37)
+    g = hws.WStackingGridder(w, w, dl, dl, 7, 63, **kw)
+    g.prepare_w_layers(nw, 0.0, 0.0, maxw)
+    g.start_inversion_pass(0)
+    g.add_data(uvw[:400], wl, vis[:400])
+    g.add_data(uvw[400:], wl, vis[400:])
+    g.finish_inversion_pass()
+    re, im = g.finalize_image(1.0)
+    for got, want in ((re[0], img0), (re[1], img1), (re[2], img2r), (im[2], img2i)):
+        assert rel(got, want) < TOL_IMG["mixed"]
+    dev = torch.device("cuda:0")
+    d = hws.DeviceGridder(w, w, dl, dl, 7, 63, **kw)
+    d.prepare_w_layers(nw, 0.0, maxw)
+    d_uvw, d_wl = torch.from_numpy(uvw).to(dev), torch.from_numpy(wl).to(dev)
+    d_vis = torch.from_numpy(vis).to(dev)
+    out = torch.zeros((3, w, w), dtype=torch.float32, device=dev)
+    out_i = torch.zeros_like(out)
+    d.plan(d_uvw, d_wl)
+    d.invert(d_vis, 1.0, out, out_i)
+    st = d.stats()
+    assert st["chunks"] > st["bins_used"] + 20  # split bins
+    o, oi = out.cpu().numpy(), out_i.cpu().numpy()
+    for got, want in ((o[0], img0), (o[1], img1), (o[2], img2r), (oi[2], img2i)):
+        assert rel(got, want) < TOL_IMG["mixed"]
+    rng = np.random.default_rng(41)
+    mre = rng.standard_normal((3, w, w)).astype(np.float32)
+    mim = rng.standard_normal((3, w, w)).astype(np.float32)
+    pred = torch.empty_like(d_vis)
+    d.predict(torch.from_numpy(mre).to(dev), torch.from_numpy(mim).to(dev), pred)
+    v_o = oracle_predict(oracle, w, w, dl, dl, uvw_l, mre[2], mim[2], nw, 0.0, maxw, cplx=True,
+                         shift=shift)
+    v_o = v_o * np.conj(ph)
+    v = pred.cpu().numpy()[:, :, 2].reshape(-1)
+    ok = ~np.isnan(v_o)
+    assert np.array_equal(np.isnan(v), ~ok)
+    assert rel(v[ok], v_o[ok]) < TOL_IMG["mixed"]
+    d.close()

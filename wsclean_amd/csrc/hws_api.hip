@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -347,12 +348,17 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
 
   // blocked layer layout where the coalesced radix-16 transforms cover the size
   g->blocked = blockedLayoutSupported(g->W, g->H);
+  {
+    const char* e = std::getenv("HWS_NO_MFMA");
+    g->noMfma = e && e[0] == '1';
+  }
   // uv tile edge of the binned forms: divides W and H, a multiple of 4, at
   // least K - 1 (a footprint then spans at most two tiles per axis) and with
-  // T + K - 1 <= 64 (6-bit record offsets)
+  // T + K - 1 <= 64 (6-bit record offsets); 16 (the matrix-core grid
+  // kernel's tile) unless the caller names one
   {
     const int K = g->K;
-    const int want = c->tile_size ? int(c->tile_size) : 32;
+    const int want = c->tile_size ? int(c->tile_size) : 16;
     int best = 0;
     for (int t = 4; t + K - 1 <= 64 && t <= 64; t += 4)
       if (g->W % t == 0 && g->H % t == 0 && t >= K - 1 && std::abs(t - want) <= std::abs(best - want))
@@ -402,12 +408,14 @@ void hws_destroy(hws_gridder* g) {
   freeDev(pl.scanTmp);
   freeDev(pl.scanScratch);
   freeDev(pl.counters);
+  freeDev(pl.queue);
   for (auto& kp : pl.kinds) {
     freeDev(kp.recSorted);
     freeDev(kp.counts);
     freeDev(kp.binStart);
     freeDev(kp.extraStart);
     freeDev(kp.itemStart);
+    freeDev(kp.itemBin);
     freeDev(kp.partials);
     freeDev(kp.keys);
     freeDev(kp.keysAlt);

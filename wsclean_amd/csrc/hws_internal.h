@@ -81,6 +81,8 @@ struct KindPlan {
   uint32_t* itemStart = nullptr;   // exclusive scan of all items per bin [nbins + 1]
   uint64_t nWorkItems = 0;
   size_t itemCapacity = 0;
+  uint32_t* itemBin = nullptr;  // bin of every work item [nWorkItems]
+  size_t itemBinCapacity = 0;
   float2* partials = nullptr;      // [nExtra][nImg][T * T]
   uint64_t nExtra = 0;
   size_t extraCapacity = 0, partialCapacity = 0;
@@ -102,6 +104,7 @@ struct Plan {
   size_t scanCapacity = 0, scanScratchCapacity = 0;
   // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}], [6 + kind]: bins used
   unsigned long long* counters = nullptr;
+  uint32_t* queue = nullptr;  // work-queue head of the running grid launch
   void* sortTemp = nullptr;  // radix sort scratch
   size_t sortTempCapacity = 0;
 };
@@ -149,6 +152,9 @@ struct hws_gridder {
   hws::FftPlan rowPlan, colPlan;
   // layer layout: blocked (four rows interleaved, see cellIndex) or natural
   bool blocked = false;
+  // grid with the VALU cell kernel at T = 16 instead of the matrix-core one
+  // (HWS_NO_MFMA=1 in the environment; for comparisons)
+  bool noMfma = false;
 
   // binning plan
   hws::Plan plan;

@@ -343,6 +343,16 @@ binItemsKernel(const uint32_t* __restrict__ binStart, const uint32_t* __restrict
     items[b] = (binStart[b + 1] > binStart[b] ? 1u : 0u) + (extraStart[b + 1] - extraStart[b]);
 }
 
+// Item -> bin table: the bin of every work item (thread per bin; a split
+// bin writes one entry per item).
+__global__ void __launch_bounds__(kThreads)
+itemBinsKernel(const uint32_t* __restrict__ itemStart, size_t nbins,
+               uint32_t* __restrict__ itemBin) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  for (uint32_t j = itemStart[b], e = itemStart[b + 1]; j < e; ++j) itemBin[j] = uint32_t(b);
+}
+
 // One work item of a pass: bin, item index within the bin (0 = the item
 // that writes the tile) and its record range. The items of the pass's bins
 // [binLo, binHi) are numbered in bin order from itemStart[binLo]; the bin of
@@ -369,10 +379,11 @@ __device__ __forceinline__ uint32_t ownerOf(const uint32_t* __restrict__ offs, u
 __device__ __forceinline__ WorkItem decodeItem(uint32_t item, uint32_t binLo, uint32_t binHi,
                                                const uint32_t* __restrict__ binStart,
                                                const uint32_t* __restrict__ itemStart,
-                                               uint32_t chunk) {
+                                               uint32_t chunk,
+                                               const uint32_t* __restrict__ itemBin = nullptr) {
   WorkItem w;
   const uint32_t g = itemStart[binLo] + item;
-  w.bin = ownerOf(itemStart, binLo, binHi, g);
+  w.bin = itemBin ? itemBin[g] : ownerOf(itemStart, binLo, binHi, g);
   w.j = g - itemStart[w.bin];
   const uint32_t s0 = binStart[w.bin], s1 = binStart[w.bin + 1];
   w.start = s0 + w.j * chunk;
@@ -532,7 +543,7 @@ template <int NIMG, int TG, bool ACCUM>
 __global__ void __launch_bounds__(kGridThreads)
 gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
                 const uint32_t* __restrict__ extraStart, const uint32_t* __restrict__ itemStart,
-                const uint2* __restrict__ recs,
+                const uint32_t* __restrict__ itemBin, const uint2* __restrict__ recs,
                 const float2* __restrict__ vis, const float* __restrict__ ktab,
                 const __half2* __restrict__ khalf, const double* __restrict__ uvw,
                 const double* __restrict__ wl, float2* __restrict__ layers,
@@ -567,7 +578,7 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   __syncthreads();
   for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
     // every lane decodes the item (uniform loads)
-    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk);
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
     const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
     if (ACCUM && count == 0) continue;  // block-uniform; nothing to add
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
@@ -731,6 +742,403 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   }
 }
 
+constexpr int kCellWaves = 4;
+constexpr int kCellThreads = 64 * kCellWaves;
+
+// Tile edges the cell-owning grid kernel takes (64 lanes cover whole rows).
+__host__ __device__ inline bool cellKernelTile(int T) { return T == 8 || T == 16; }
+
+// Row length of the zero-padded kernel table of the cell kernel: K values
+// with T - 1 zeros on each side, so that any tap offset a cell of the tile
+// can have from a footprint start reads either a kernel value or a zero.
+__host__ __device__ inline int paddedRow(int K, int T) { return K + 2 * (T - 1); }
+
+// The padded table is used when it fits in 32 KiB of LDS (OS = 63 at K = 7);
+// larger oversampling uses the compact table with clamped indexes.
+__host__ __device__ inline bool cellTablePadded(int K, int OS, int T) {
+  return size_t(OS) * paddedRow(K, T) * sizeof(float) <= 32768;
+}
+
+// Gridding of one image kind, one work item per WAVE: the wave owns the T x T
+// tile of the item's (layer, uv tile) bin with lane l holding the cells of
+// column l mod T and R consecutive rows from (l div T) * R, so 64 lanes cover
+// the tile (T = 16: 4 rows per lane; T = 8: 1). Records are loaded 64 at a
+// time, one per lane, into registers (kernel-table offsets, rotated and
+// conjugated visibilities) and broadcast to the wave one by one through
+// scalar registers; every lane then adds to its own cells the taps of the
+// record's K x K footprint that fall on them, kernel value
+// yKernel[dy] * xKernel[dx] (gridding/wstackinggridder.cpp:560-597). With
+// the zero-padded table (PAD) a cell outside the footprint reads a zero, so
+// the loop has no tests at all. ROT applies the phase-centre rotation (a
+// separate instance: its FP64 sincos costs registers). The sums stay in
+// registers for the whole
+// item: no LDS traffic but the kernel table, no read-modify-write chains, no
+// atomics. Item 0 of a bin writes the tile to the layer (ACCUM adds to it);
+// extra items of a split bin write partial tiles that reducePartialsKernel
+// adds in item order.
+template <int NIMG, int T, bool ACCUM, bool PAD, bool ROT>
+__global__ void __launch_bounds__(kCellThreads)
+gridCellsKernel(GridArgs a, const uint32_t* __restrict__ binStart,
+                const uint32_t* __restrict__ extraStart, const uint32_t* __restrict__ itemStart,
+                const uint32_t* __restrict__ itemBin, const uint2* __restrict__ recs,
+                const float2* __restrict__ vis, const float* __restrict__ ktab,
+                const __half2* __restrict__ khalf, const double* __restrict__ uvw,
+                const double* __restrict__ wl, float2* __restrict__ layers,
+                float2* __restrict__ partials, uint32_t* __restrict__ queue) {
+  constexpr int G = 64 / T;  // lane row groups
+  constexpr int R = T / G;   // rows per lane
+  static_assert(G * T == 64 && G * R == T, "64 lanes must cover the tile");
+  constexpr int T2 = T * T;
+  extern __shared__ float smem[];
+  const int K = a.ga.K, OS = a.ga.OS;
+  float* kt = smem;  // PAD: [OS][K + 2 (T - 1)], else [OS][K]
+  const int rowLen = PAD ? paddedRow(K, T) : K;
+  if (PAD) {
+    for (int i = threadIdx.x; i < OS * rowLen; i += blockDim.x) {
+      const int row = i / rowLen, c = i - row * rowLen - (T - 1);
+      float v = 0.0f;
+      if (c >= 0 && c < K) {
+        const int src = row * K + c;
+        if (khalf) {
+          const __half2 h = khalf[src];
+          v = __low2float(h) + __high2float(h);
+        } else {
+          v = ktab[src];
+        }
+      }
+      kt[i] = v;
+    }
+  } else {
+    loadKernelTable(kt, ktab, khalf, OS * K);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cx = lane % T, cy0 = (lane / T) * R;
+  const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
+  const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
+  const uint32_t nItems = passItems(binLo, binHi, itemStart);
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  // items come from a work queue (a counter the host clears before the
+  // launch): a wave takes the next item when it finishes one, so a few
+  // heavy items do not leave most waves idle at the end. The next index is
+  // requested while the current item runs.
+  (void)wave;
+  auto take = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(queue, 1u);
+    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+  };
+  uint32_t next = take();
+  while (next < nItems) {
+    const uint32_t item = next;
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
+    next = take();
+    const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
+    if (ACCUM && count == 0) continue;
+    float acc[R][2 * NIMG];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int e = 0; e < 2 * NIMG; ++e) acc[r][e] = 0.0f;
+    for (uint32_t b0 = 0; b0 < count; b0 += 64u) {
+      // lane l loads record b0 + l; lanes past the end hold a zero sample
+      // (PAD: table offsets of a zero row region; else position and rows)
+      int pos = 0, kof = 0;
+      float2 v[NIMG];
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) v[gi] = make_float2(0.0f, 0.0f);
+      const uint32_t it = b0 + uint32_t(lane);
+      if (it < count) {
+        const uint2 rec = recs[start + it];
+        const uint32_t idx = rec.x & 0x3FFFFFFFu;
+        const bool conj = (rec.x >> 31) != 0;
+        const int kk = int(rec.y >> 12);
+        const int ky = kk / OS, kx = kk - ky * OS;
+        const int xs = int(rec.y & 63u), ys = int((rec.y >> 6) & 63u);  // origin + K - 1
+        if (PAD) {
+          // table index of tap (cx, cy) = row base + (T - 1) + (c - origin)
+          pos = (kx * rowLen + (T - 1) + (K - 1) - xs) |
+                ((ky * rowLen + (T - 1) + (K - 1) - ys) << 16);
+        } else {
+          pos = xs | (ys << 6);
+          kof = (ky * K) | (kx * K) << 16;
+        }
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+          if (ROT) x = rotateSample(x, a, uvw, wl, idx, -1.0);
+          if (conj) x.y = -x.y;
+          v[gi] = x;
+        }
+      }
+      const int nb = int(min(64u, count - b0));
+      for (int u0 = 0; u0 < nb; u0 += 4) {
+        // four records at a time: every kernel-table read is unconditional,
+        // so the reads of all four issue before the first wait
+        float kx[4], ky[4][R];
+        bool inx[4], iny[4][R];
+#pragma unroll
+        for (int du = 0; du < 4; ++du) {
+          const int u = u0 + du;
+          const int ps = __builtin_amdgcn_readlane(pos, u);
+          if (PAD) {
+            const int ax = (ps & 0xFFFF) + cx, ay = (ps >> 16) + cy0;
+            kx[du] = kt[ax];
+#pragma unroll
+            for (int r = 0; r < R; ++r) ky[du][r] = kt[ay + r];
+          } else {
+            const int ko = __builtin_amdgcn_readlane(kof, u);
+            const int xo = (ps & 63) - (K - 1), yo = (ps >> 6) - (K - 1);
+            const int kyo = ko & 0xFFFF, kxo = ko >> 16;
+            const int dx = cx - xo;
+            inx[du] = unsigned(dx) < unsigned(K);
+            kx[du] = kt[kxo + min(max(dx, 0), K - 1)];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const int dy = cy0 + r - yo;
+              iny[du][r] = unsigned(dy) < unsigned(K);
+              ky[du][r] = kt[kyo + min(max(dy, 0), K - 1)];
+            }
+          }
+        }
+#pragma unroll
+        for (int du = 0; du < 4; ++du) {
+          const int u = u0 + du;
+          float2 vv[NIMG];
+#pragma unroll
+          for (int gi = 0; gi < NIMG; ++gi) {
+            vv[gi].x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[gi].x), u));
+            vv[gi].y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[gi].y), u));
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            // yKernelValue * xKernel[i] (gridding/wstackinggridder.cpp:574)
+            float kv = ky[du][r] * kx[du];
+            if (!PAD) kv = (inx[du] && iny[du][r]) ? kv : 0.0f;
+#pragma unroll
+            for (int gi = 0; gi < NIMG; ++gi) {
+              acc[r][2 * gi] += vv[gi].x * kv;
+              acc[r][2 * gi + 1] += vv[gi].y * kv;
+            }
+          }
+        }
+      }
+    }
+    if (wi.j > 0) {
+      // an extra item of a large bin: its tile goes to the partial slot
+      float2* dst = partials + size_t(extraStart[bin] + wi.j - 1u) * size_t(NIMG * T2);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi)
+          dst[gi * T2 + (cy0 + r) * T + cx] = make_float2(acc[r][2 * gi], acc[r][2 * gi + 1]);
+    } else {
+      const int layer = int(bin / uint32_t(a.tilesPerLayer));
+      const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
+      const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+      const int x = tx * T + cx, y0 = ty * T + cy0;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        float2* base = layers + (size_t(a.imgs.idx[gi]) * a.nLayersPass +
+                                 size_t(layer - a.passStart)) * HW;
+        if (R == 4 && a.ga.blocked) {
+          // rows y0 .. y0 + 3 of one column are 32 contiguous bytes
+          float4* dst = reinterpret_cast<float4*>(base + cellIndex(x, y0, a.ga.W, 1));
+          float4 lo = make_float4(acc[0][2 * gi], acc[0][2 * gi + 1], acc[1 % R][2 * gi],
+                                  acc[1 % R][2 * gi + 1]);
+          float4 hi = make_float4(acc[2 % R][2 * gi], acc[2 % R][2 * gi + 1],
+                                  acc[3 % R][2 * gi], acc[3 % R][2 * gi + 1]);
+          if (ACCUM) {
+            const float4 o0 = dst[0], o1 = dst[1];
+            lo.x += o0.x; lo.y += o0.y; lo.z += o0.z; lo.w += o0.w;
+            hi.x += o1.x; hi.y += o1.y; hi.z += o1.z; hi.w += o1.w;
+          }
+          dst[0] = lo;
+          dst[1] = hi;
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            float2* dst = base + cellIndex(x, y0 + r, a.ga.W, a.ga.blocked);
+            float2 val = make_float2(acc[r][2 * gi], acc[r][2 * gi + 1]);
+            if (ACCUM) {
+              const float2 o = *dst;
+              val.x += o.x;
+              val.y += o.y;
+            }
+            *dst = val;
+          }
+        }
+      }
+    }
+  }
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// The tile of a bin as a matrix product: with A[y][r] = v_r * yKernel_r[y -
+// y0_r] and B[r][x] = xKernel_r[x - x0_r] (zero outside record r's
+// footprint), the gridded 16 x 16 tile is G = sum_r A[.][r] B[r][.] = A B,
+// a rank-K update over the item's records, done four records at a time by
+// v_mfma_f32_16x16x4_f32 (FP32 products and sums) per image component.
+// Operand layout of that instruction: lane i + 16 k supplies A[i][k] and
+// B[k][i]; lane j + 16 (i / 4) receives D[i][j] in register i % 4, which is
+// the cell ownership of gridCellsKernel at T = 16 (column lane mod 16, rows
+// 4 (lane div 16) ..+3). Records are staged in the wave's LDS slice 64 at a
+// time; lane i + 16 k reads record u0 + k of each group of four and looks
+// up its own row (ky) and column (kx) in the zero-padded kernel table.
+// Work items come from the work queue as in gridCellsKernel.
+constexpr int kMfmaWaves = 4;
+constexpr int kMfmaThreads = 64 * kMfmaWaves;
+
+template <int NIMG, bool ACCUM, bool ROT>
+__global__ void __launch_bounds__(kMfmaThreads)
+gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
+               const uint32_t* __restrict__ extraStart, const uint32_t* __restrict__ itemStart,
+               const uint32_t* __restrict__ itemBin, const uint2* __restrict__ recs,
+               const float2* __restrict__ vis, const float* __restrict__ ktab,
+               const __half2* __restrict__ khalf, const double* __restrict__ uvw,
+               const double* __restrict__ wl, float2* __restrict__ layers,
+               float2* __restrict__ partials, uint32_t* __restrict__ queue) {
+  constexpr int T = 16, T2 = T * T;
+  extern __shared__ float smem[];
+  const int K = a.ga.K, OS = a.ga.OS;
+  const int rowLen = paddedRow(K, T);
+  float* kt = smem;  // [OS][K + 2 (T - 1)], zero padded
+  const int tableFloats = (OS * rowLen + 3) & ~3;
+  int* stPos = reinterpret_cast<int*>(smem + tableFloats);                 // [wave][64]
+  float2* stVis = reinterpret_cast<float2*>(stPos + kMfmaWaves * 64);      // [wave][64][NIMG]
+  for (int i = threadIdx.x; i < OS * rowLen; i += blockDim.x) {
+    const int row = i / rowLen, c = i - row * rowLen - (T - 1);
+    float v = 0.0f;
+    if (c >= 0 && c < K) {
+      const int src = row * K + c;
+      if (khalf) {
+        const __half2 h = khalf[src];
+        v = __low2float(h) + __high2float(h);
+      } else {
+        v = ktab[src];
+      }
+    }
+    kt[i] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;  // operand row/column and record slot
+  int* myPos = stPos + wave * 64;
+  float2* myVis = stVis + wave * 64 * NIMG;
+  const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
+  const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
+  const uint32_t nItems = passItems(binLo, binHi, itemStart);
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  auto take = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(queue, 1u);
+    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+  };
+  uint32_t next = take();
+  while (next < nItems) {
+    const uint32_t item = next;
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
+    next = take();
+    const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
+    if (ACCUM && count == 0) continue;
+    floatx4 acc[2 * NIMG];
+#pragma unroll
+    for (int e = 0; e < 2 * NIMG; ++e) acc[e] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t b0 = 0; b0 < count; b0 += 64u) {
+      // stage 64 records (zero samples past the end: their A rows vanish)
+      int pos = 0;
+      float2 v[NIMG];
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) v[gi] = make_float2(0.0f, 0.0f);
+      const uint32_t it = b0 + uint32_t(lane);
+      if (it < count) {
+        const uint2 rec = recs[start + it];
+        const uint32_t idx = rec.x & 0x3FFFFFFFu;
+        const bool conj = (rec.x >> 31) != 0;
+        const int kk = int(rec.y >> 12);
+        const int ky = kk / OS, kx = kk - ky * OS;
+        const int xs = int(rec.y & 63u), ys = int((rec.y >> 6) & 63u);
+        pos = (kx * rowLen + (T - 1) + (K - 1) - xs) |
+              ((ky * rowLen + (T - 1) + (K - 1) - ys) << 16);
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+          if (ROT) x = rotateSample(x, a, uvw, wl, idx, -1.0);
+          if (conj) x.y = -x.y;
+          v[gi] = x;
+        }
+      }
+      myPos[lane] = pos;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) myVis[lane * NIMG + gi] = v[gi];
+      waveSync();
+      const int nb = int(min(64u, count - b0));
+      for (int u0 = 0; u0 < nb; u0 += 4) {
+        const int r = u0 + lk;
+        const int ps = myPos[r];
+        // B[k][x = li] = xKernel, A[y = li][k] = v * yKernel
+        // (yKernelValue * xKernel[i], gridding/wstackinggridder.cpp:574)
+        const float bx = kt[(ps & 0xFFFF) + li];
+        const float ky = kt[(ps >> 16) + li];
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          const float2 vv = myVis[r * NIMG + gi];
+          acc[2 * gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(vv.x * ky, bx, acc[2 * gi], 0, 0, 0);
+          acc[2 * gi + 1] =
+              __builtin_amdgcn_mfma_f32_16x16x4f32(vv.y * ky, bx, acc[2 * gi + 1], 0, 0, 0);
+        }
+      }
+      waveSync();  // the staging is rewritten next batch
+    }
+    // lane owns column li, rows 4 lk .. 4 lk + 3
+    if (wi.j > 0) {
+      float2* dst = partials + size_t(extraStart[bin] + wi.j - 1u) * size_t(NIMG * T2);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi)
+          dst[gi * T2 + (4 * lk + r) * T + li] = make_float2(acc[2 * gi][r], acc[2 * gi + 1][r]);
+    } else {
+      const int layer = int(bin / uint32_t(a.tilesPerLayer));
+      const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
+      const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+      const int x = tx * T + li, y0 = ty * T + 4 * lk;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        float2* base = layers + (size_t(a.imgs.idx[gi]) * a.nLayersPass +
+                                 size_t(layer - a.passStart)) * HW;
+        const floatx4 re = acc[2 * gi], im = acc[2 * gi + 1];
+        if (a.ga.blocked) {
+          // rows y0 .. y0 + 3 of one column are 32 contiguous bytes
+          float4* dst = reinterpret_cast<float4*>(base + cellIndex(x, y0, a.ga.W, 1));
+          float4 lo = make_float4(re[0], im[0], re[1], im[1]);
+          float4 hi = make_float4(re[2], im[2], re[3], im[3]);
+          if (ACCUM) {
+            const float4 o0 = dst[0], o1 = dst[1];
+            lo.x += o0.x; lo.y += o0.y; lo.z += o0.z; lo.w += o0.w;
+            hi.x += o1.x; hi.y += o1.y; hi.z += o1.z; hi.w += o1.w;
+          }
+          dst[0] = lo;
+          dst[1] = hi;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float2* dst = base + cellIndex(x, y0 + r, a.ga.W, 0);
+            float2 val = make_float2(re[r], im[r]);
+            if (ACCUM) {
+              const float2 o = *dst;
+              val.x += o.x;
+              val.y += o.y;
+            }
+            *dst = val;
+          }
+        }
+      }
+    }
+  }
+}
+
 // Degridding of one image kind, one work item per workgroup at a time: the
 // (T + K - 1)^2 subgrid around the item's (layer, tile) is gathered from the
 // layer into LDS; each lane then sums the K x K taps of one primary record
@@ -739,7 +1147,7 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
 degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
-                  const uint32_t* __restrict__ itemStart,
+                  const uint32_t* __restrict__ itemStart, const uint32_t* __restrict__ itemBin,
                   const uint2* __restrict__ recs, const float* __restrict__ ktab,
                   const __half2* __restrict__ khalf, const double* __restrict__ uvw,
                   const double* __restrict__ wl, const float2* __restrict__ layers,
@@ -755,7 +1163,7 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   const uint32_t nItems = passItems(binLo, binHi, itemStart);
   for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
     // every lane decodes the item (uniform loads)
-    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk);
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
     const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
     if (count == 0) continue;  // block-uniform
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
@@ -1348,12 +1756,27 @@ ImgArgs imgArgs(const hws_gridder* g, int pass) {
   return a;
 }
 
+// The matrix-core grid kernel covers 16 x 16 tiles with the padded table.
+bool mfmaKernelUsable(const hws_gridder* g) {
+  return g->plan.tileSize == 16 && cellTablePadded(g->K, g->OS, 16) && !g->noMfma;
+}
+
 size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
   const int T = g->tileSize;
   const size_t kt = size_t((g->OS * g->K + 3) & ~3) * sizeof(float);
   if (degrid) {
     const int sg = T + g->K - 1;
     return kt + size_t(nimg) * sg * sg * sizeof(float2);
+  }
+  if (mfmaKernelUsable(g)) {
+    const size_t table = size_t((g->OS * paddedRow(g->K, T) + 3) & ~3) * sizeof(float);
+    return table + kMfmaWaves * 64 * (sizeof(int) + size_t(nimg) * sizeof(float2));
+  }
+  if (cellKernelTile(T)) {
+    // the cell kernel keeps its tile in registers; LDS holds the table only
+    if (cellTablePadded(g->K, g->OS, T))
+      return size_t(g->OS) * paddedRow(g->K, T) * sizeof(float);
+    return kt;
   }
   // per wave: a private T x pitch tile per image component + 64 staged records
   const size_t perWave = size_t(nimg) * 2 * T * privatePitch(T) * sizeof(float) +
@@ -1422,6 +1845,7 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                       "plan scan scratch"));
   if (!pl.counters)
     HWS_TRY(checkHip(g, hipMalloc(&pl.counters, 8 * sizeof(unsigned long long)), "plan counters"));
+  if (!pl.queue) HWS_TRY(checkHip(g, hipMalloc(&pl.queue, 64 * sizeof(uint32_t)), "plan queue"));
   HWS_TRY(launchFill(g, pl.counters, 0u, 16));
 
   const GeomArgs ga = geomArgs(g);
@@ -1472,6 +1896,11 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     kp.nItems = total;
     kp.nExtra = totals[1];
     kp.nWorkItems = totals[2];
+    HWS_TRY(ensureAlloc(g, kp.itemBin, kp.itemBinCapacity, size_t(kp.nWorkItems),
+                        "plan item bins"));
+    hipLaunchKernelGGL(itemBinsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
+                       kp.itemStart, pl.nbins, kp.itemBin);
+    HWS_TRY(checkHip(g, hipGetLastError(), "item bin launch"));
     HWS_TRY(ensureAlloc(g, kp.recSorted, kp.capacity, size_t(total), "plan sorted records"));
     if (kp.nExtra)
       HWS_TRY(ensureAlloc(g, kp.partials, kp.partialCapacity,
@@ -1485,7 +1914,9 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     // sort keys bin << 12 | origin need 12 + log2(nbins) <= 32 bits
     int binBits = 0;
     while ((size_t(1) << binBits) < pl.nbins) ++binBits;
-    const bool sortByOrigin = binBits <= 20 && total > 1;
+    // (the cell-owning grid kernel sums every record in registers, in any
+    // order: it needs no sort)
+    const bool sortByOrigin = binBits <= 20 && total > 1 && !cellKernelTile(pl.tileSize);
     if (sortByOrigin) {
       HWS_TRY(ensureAlloc(g, kp.keys, kp.keyCapacity, size_t(total), "plan sort keys"));
       HWS_TRY(ensureAlloc(g, kp.keysAlt, kp.keyAltCapacity, size_t(total), "plan sort keys"));
@@ -1567,9 +1998,82 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
         HWS_TRY(checkHip(g, hipGetLastError(), "layer clear launch"));
       }
       // partial tiles of a launch hold its own images only
+      if (mfmaKernelUsable(g)) {
+        const unsigned mblocks = unsigned(std::max<uint64_t>(
+            1, std::min<uint64_t>((kp.nWorkItems + kMfmaWaves - 1) / kMfmaWaves,
+                                  uint64_t(persistentBlocks(g, 8)))));
+        hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, g->stream, pl.queue, 0u, size_t(1));
+#define HWS_MFMA_GO(N, ACC, ROT)                                                               \
+  hipLaunchKernelGGL((gridMfmaKernel<N, ACC, ROT>), dim3(mblocks), dim3(kMfmaThreads), lds,     \
+                     g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.itemBin,        \
+                     kp.recSorted, vis, g->dKernel, khalf, pl.uvw, pl.wavelengths, layers,      \
+                     kp.partials, pl.queue)
+#define HWS_MFMA_CASE(N)                                                                       \
+  case N:                                                                                      \
+    if (accumulate) {                                                                          \
+      if (a.ga.rotate) HWS_MFMA_GO(N, true, true); else HWS_MFMA_GO(N, true, false);          \
+    } else {                                                                                   \
+      if (a.ga.rotate) HWS_MFMA_GO(N, false, true); else HWS_MFMA_GO(N, false, false);        \
+    }                                                                                          \
+    break;
+        switch (n) {
+          HWS_MFMA_CASE(1)
+          HWS_MFMA_CASE(2)
+          HWS_MFMA_CASE(3)
+          HWS_MFMA_CASE(4)
+        }
+#undef HWS_MFMA_CASE
+#undef HWS_MFMA_GO
+      } else if (cellKernelTile(pl.tileSize)) {
+        const bool pad = cellTablePadded(g->K, g->OS, pl.tileSize);
+        const unsigned cblocks = unsigned(std::max<uint64_t>(
+            1, std::min<uint64_t>((kp.nWorkItems + kCellWaves - 1) / kCellWaves,
+                                  uint64_t(persistentBlocks(g, 8)))));
+#define HWS_CELL_GO(N, TT, ACC, PAD, ROT)                                                   \
+  hipLaunchKernelGGL((gridCellsKernel<N, TT, ACC, PAD, ROT>), dim3(cblocks), dim3(kCellThreads), \
+                     lds, g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.itemBin,    \
+                     kp.recSorted, vis, g->dKernel, khalf, pl.uvw, pl.wavelengths, layers,       \
+                     kp.partials, pl.queue)
+#define HWS_CELL_LAUNCH(N, TT, ACC)                \
+  do {                                             \
+    hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, g->stream, pl.queue, 0u, size_t(1)); \
+    if (pad && !a.ga.rotate)                       \
+      HWS_CELL_GO(N, TT, ACC, true, false);        \
+    else if (pad)                                  \
+      HWS_CELL_GO(N, TT, ACC, true, true);         \
+    else if (!a.ga.rotate)                         \
+      HWS_CELL_GO(N, TT, ACC, false, false);       \
+    else                                           \
+      HWS_CELL_GO(N, TT, ACC, false, true);        \
+  } while (0)
+#define HWS_CELL_T(N, ACC)             \
+  if (pl.tileSize == 16)               \
+    HWS_CELL_LAUNCH(N, 16, ACC);       \
+  else                                 \
+    HWS_CELL_LAUNCH(N, 8, ACC);
+#define HWS_CELL_CASE(N)               \
+  case N:                              \
+    if (accumulate) {                  \
+      HWS_CELL_T(N, true)              \
+    } else {                           \
+      HWS_CELL_T(N, false)             \
+    }                                  \
+    break;
+        switch (n) {
+          HWS_CELL_CASE(1)
+          HWS_CELL_CASE(2)
+          HWS_CELL_CASE(3)
+          HWS_CELL_CASE(4)
+        }
+#undef HWS_CELL_CASE
+#undef HWS_CELL_T
+#undef HWS_CELL_LAUNCH
+#undef HWS_CELL_GO
+      } else {
 #define HWS_GRID_LAUNCH(N, TG, ACC)                                                           \
   hipLaunchKernelGGL((gridTilesKernel<N, TG, ACC>), dim3(blocks), dim3(kGridThreads), lds,      \
-                     g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.recSorted, vis, \
+                     g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.itemBin,        \
+                     kp.recSorted, vis,                                                         \
                      g->dKernel,                                                                \
                      khalf, pl.uvw, pl.wavelengths, layers, kp.partials)
 #define HWS_GRID_TG(N, ACC)          \
@@ -1598,6 +2102,7 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
 #undef HWS_GRID_CASE
 #undef HWS_GRID_TG
 #undef HWS_GRID_LAUNCH
+      }
       HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
       if (kp.nExtra) {
         const uint64_t groups = (uint64_t(pl.tileSize) * pl.tileSize + kThreads - 1) / kThreads;
@@ -1638,7 +2143,8 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
 #define HWS_DEGRID_CASE(N)                                                                 \
   case N:                                                                                  \
     hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream,  \
-                       a, kp.binStart, kp.itemStart, kp.recSorted, g->dKernel, khalf, pl.uvw, \
+                       a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted, g->dKernel,    \
+                       khalf, pl.uvw,                                                      \
                        pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out);  \
     break;
       HWS_DEGRID_CASE(1)
