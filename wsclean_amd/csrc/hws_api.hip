@@ -197,12 +197,33 @@ int layersInPass(const hws_gridder* g, int pass) {
   return passLayerStart(g, pass + 1) - passLayerStart(g, pass);
 }
 
-hws_status fftPassLayers(hws_gridder* g, int pass, int sign) {
+// sparse: the pass's layers hold only the tiles of bins with records (the
+// binned grid without the clear); the backward transform skips the others.
+hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false) {
   const int n = layersInPass(g, pass);
   const size_t HW = size_t(g->W) * g->H;
-  for (int img = 0; img < g->nImg; ++img)
-    HWS_TRY(fft2d(g, g->dLayers + size_t(img) * g->maxLayersPerPass * HW, n, sign));
+  for (int img = 0; img < g->nImg; ++img) {
+    FftSparse sp{};
+    if (sparse) {
+      const Plan& pl = g->plan;
+      sp.occ = pl.kinds[g->isComplex[img] ? 1 : 0].occ;
+      sp.layer0 = passLayerStart(g, pass);
+      sp.tilesY = pl.tilesY;
+      sp.tshift = 0;
+      while ((1 << sp.tshift) < pl.tileSize) ++sp.tshift;
+    }
+    HWS_TRY(fft2d(g, g->dLayers + size_t(img) * g->maxLayersPerPass * HW, n, sign,
+                  sparse ? &sp : nullptr));
+  }
   return HWS_OK;
+}
+
+// The binned invert leaves empty tiles unwritten when the blocked transform
+// can skip them (power-of-two tile edge of at least 4).
+bool sparseInvert(const hws_gridder* g) {
+  const int T = g->plan.tileSize;
+  return g->blocked && g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && T >= 4 &&
+         (T & (T - 1)) == 0;
 }
 
 bool prepared(const hws_gridder* g) { return g->nPasses > 0 && g->dLayers; }
@@ -416,6 +437,7 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.extraStart);
     freeDev(kp.itemStart);
     freeDev(kp.itemBin);
+    freeDev(kp.occ);
     freeDev(kp.partials);
     freeDev(kp.keys);
     freeDev(kp.keysAlt);
@@ -735,12 +757,14 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
       g->layersZero = false;
       HWS_TRY(gridPlain(g, pl.nrow, pl.nchan, pl.uvw, pl.wavelengths, d_vis, pass));
     } else {
-      // the pass's layers are cleared and the non-empty tiles written
+      // the non-empty tiles are written (and the rest cleared unless the
+      // transform skips them)
       g->layersZero = false;
-      HWS_TRY(gridBinned(g, d_vis, pass, false));
+      HWS_TRY(gridBinned(g, d_vis, pass, false, sparseInvert(g)));
     }
     HWS_TRY(record(g, 3));
-    HWS_TRY(fftPassLayers(g, pass, +1));
+    HWS_TRY(fftPassLayers(g, pass, +1,
+                          g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && sparseInvert(g)));
     HWS_TRY(record(g, 4));
     HWS_TRY(projectPass(g, pass));
     HWS_TRY(record(g, 5));

@@ -283,9 +283,13 @@ __device__ __forceinline__ size_t blockedAddr(int pos, int f, int W) {
   return (size_t(pos >> 2) * W + f) * 4 + (pos & 3);
 }
 
+// live (rows pass only, may be null): one flag per uv tile of the row block;
+// cells of a tile without samples read as zero (their memory is not read,
+// and need not have been cleared). tshift = log2 of the tile edge.
 template <int SIGN, int R, int N, int NS, bool ROWS>
 __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g, int W,
-                                             const float2* __restrict__ tw, int tid) {
+                                             const float2* __restrict__ tw, int tid,
+                                             const unsigned char* live, int tshift) {
   constexpr int B = N / R;
   constexpr int T = N / 4;
   constexpr int PER = 16 / R;
@@ -301,7 +305,12 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int pos = b + r * B;
-      float2 x = FIRST ? g[blockedAddr<ROWS>(pos, f, W)] : lds[padIdx(pos * 4 + f)];
+      float2 x;
+      if (FIRST)
+        x = (ROWS && live && !live[pos >> tshift]) ? make_float2(0.0f, 0.0f)
+                                                    : g[blockedAddr<ROWS>(pos, f, W)];
+      else
+        x = lds[padIdx(pos * 4 + f)];
       if (!FIRST && r != 0) x = cmul(x, tw[jm * r * TWSTEP]);
       v[q][r] = x;
     }
@@ -328,13 +337,15 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
 
 template <int SIGN, int N, bool ROWS, int NS = 1>
 __device__ __forceinline__ void blockedFft(float2* lds, float2* __restrict__ g, int W,
-                                           const float2* __restrict__ tw, int tid) {
+                                           const float2* __restrict__ tw, int tid,
+                                           const unsigned char* live = nullptr,
+                                           int tshift = 0) {
   if constexpr (NS < N) {
     constexpr int REM = N / NS;
     constexpr int R = (REM % 16 == 0) ? 16 : REM;  // 16s, then the remaining 8, 4 or 2
     static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix");
-    blockedStage<SIGN, R, N, NS, ROWS>(lds, g, W, tw, tid);
-    blockedFft<SIGN, N, ROWS, NS * R>(lds, g, W, tw, tid);
+    blockedStage<SIGN, R, N, NS, ROWS>(lds, g, W, tw, tid, live, tshift);
+    blockedFft<SIGN, N, ROWS, NS * R>(lds, g, W, tw, tid, live, tshift);
   }
 }
 
@@ -345,6 +356,30 @@ fftRowsBlockedKernel(float2* __restrict__ data, int H, const float2* __restrict_
   __shared__ float2 lds[4 * N + (4 * N) / 64];
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
   blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x);
+}
+
+// The same over layers whose uv tiles without samples were never written:
+// occ holds one bit per (layer, tile row, tile) bin of the layer set
+// (bin = (layer * tilesY + ty) * tilesX + tx), layer0 is the layer of batch
+// entry 0, the tile edge is 1 << tshift (>= 4, so the four rows of a row
+// block share one tile row).
+template <int SIGN, int N>
+__global__ void __launch_bounds__(N / 4)
+fftRowsBlockedSparseKernel(float2* __restrict__ data, int H, const float2* __restrict__ tw,
+                           const uint32_t* __restrict__ occ, int layer0, int tilesY,
+                           int tshift) {
+  __shared__ float2 lds[4 * N + (4 * N) / 64];
+  __shared__ unsigned char live[N / 4];
+  const int tilesX = N >> tshift;
+  const int ty = (int(blockIdx.x) * 4) >> tshift;
+  const size_t base = (size_t(layer0 + int(blockIdx.y)) * tilesY + ty) * tilesX;
+  for (int t = threadIdx.x; t < tilesX; t += blockDim.x) {
+    const size_t b = base + t;
+    live[t] = (occ[b >> 5] >> (b & 31)) & 1u;
+  }
+  __syncthreads();
+  float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
+  blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x, live, tshift);
 }
 
 // grid (W / 4, nbatch): block = columns 4*cb .. 4*cb+3 of layer blockIdx.y;
@@ -584,14 +619,33 @@ bool blockedLayoutSupported(int W, int H) {
   return ok(W) && ok(H);
 }
 
-hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign) {
+hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSparse* sparse) {
   if (nbatch <= 0) return HWS_OK;
   const int W = g->W, H = g->H;
   if (g->blocked) {
     const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
     const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
     bool ok;
-    if (sign > 0)
+    if (sign > 0 && sparse) {
+      ok = false;
+      switch (W) {
+#define HWS_SPARSE_CASE(NN)                                                                    \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL((fftRowsBlockedSparseKernel<+1, NN>), dim3(H / 4, nbatch), dim3(NN / 4), \
+                       0, g->stream, data, H, twRow, sparse->occ, sparse->layer0,              \
+                       sparse->tilesY, sparse->tshift);                                        \
+    ok = true;                                                                                 \
+    break;
+        HWS_SPARSE_CASE(256)
+        HWS_SPARSE_CASE(512)
+        HWS_SPARSE_CASE(1024)
+        HWS_SPARSE_CASE(2048)
+        HWS_SPARSE_CASE(4096)
+#undef HWS_SPARSE_CASE
+        default: break;
+      }
+      ok = ok && launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);
+    } else if (sign > 0)
       ok = launchBlocked<+1>(W, true, data, W, H, nbatch, twRow, g->stream) &&
            launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);
     else

@@ -83,6 +83,8 @@ struct KindPlan {
   size_t itemCapacity = 0;
   uint32_t* itemBin = nullptr;  // bin of every work item [nWorkItems]
   size_t itemBinCapacity = 0;
+  uint32_t* occ = nullptr;  // bit per bin: the bin holds records [(nbins + 31) / 32]
+  size_t occCapacity = 0;
   float2* partials = nullptr;      // [nExtra][nImg][T * T]
   uint64_t nExtra = 0;
   size_t extraCapacity = 0, partialCapacity = 0;
@@ -185,7 +187,10 @@ hws_status checkHip(hws_gridder* g, hipError_t e, const char* what);
 hws_status launchFill(hws_gridder* g, void* ptr, uint32_t value, size_t n32);
 hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl);
-hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate);
+// sparseLayers: tiles without samples are left unwritten (not cleared); the
+// caller reads the layers through the kinds' occupancy bits only.
+hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate,
+                      bool sparseLayers = false);
 hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass);
 // Sorts n records (and their 32-bit keys) by key bits [0, endBit) on the
 // context's stream; on return `recs` holds the sorted records (the two record
@@ -209,7 +214,17 @@ hws_status imageToLayers(hws_gridder* g, int pass);
 hws_status makeFftPlan(hws_gridder* g, int n, FftPlan& plan);
 void freeFftPlan(FftPlan& plan);
 // Unnormalised 2-D transform of nbatch H x W complex arrays, in place.
-hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign);
+// Uv tiles without samples in the layers of a backward (inverse) transform:
+// their memory is skipped (not read, need not be cleared). Blocked layout
+// only; see fftRowsBlockedSparseKernel.
+struct FftSparse {
+  const uint32_t* occ;  // one bit per (layer, tile) bin
+  int layer0;           // layer of the first transform of the batch
+  int tilesY;
+  int tshift;           // log2 of the tile edge
+};
+hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign,
+                 const FftSparse* sparse = nullptr);
 
 // Storage index of layer cell (x, y): natural row-major, or blocked with
 // groups of four rows interleaved element by element.

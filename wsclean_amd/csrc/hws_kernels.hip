@@ -353,6 +353,20 @@ itemBinsKernel(const uint32_t* __restrict__ itemStart, size_t nbins,
   for (uint32_t j = itemStart[b], e = itemStart[b + 1]; j < e; ++j) itemBin[j] = uint32_t(b);
 }
 
+// Occupancy bits: bit b of the array is set when bin b holds records
+// (thread per 32-bit word).
+__global__ void __launch_bounds__(kThreads)
+occupancyKernel(const uint32_t* __restrict__ binStart, size_t nbins,
+                uint32_t* __restrict__ occ) {
+  const size_t w = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t b0 = w * 32;
+  if (b0 >= nbins) return;
+  uint32_t bits = 0;
+  for (int j = 0; j < 32 && b0 + j < nbins; ++j)
+    if (binStart[b0 + j + 1] > binStart[b0 + j]) bits |= 1u << j;
+  occ[w] = bits;
+}
+
 // One work item of a pass: bin, item index within the bin (0 = the item
 // that writes the tile) and its record range. The items of the pass's bins
 // [binLo, binHi) are numbered in bin order from itemStart[binLo]; the bin of
@@ -1898,6 +1912,11 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     kp.nWorkItems = totals[2];
     HWS_TRY(ensureAlloc(g, kp.itemBin, kp.itemBinCapacity, size_t(kp.nWorkItems),
                         "plan item bins"));
+    const size_t occWords = (pl.nbins + 31) / 32;
+    HWS_TRY(ensureAlloc(g, kp.occ, kp.occCapacity, occWords, "plan occupancy"));
+    hipLaunchKernelGGL(occupancyKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
+                       kp.binStart, pl.nbins, kp.occ);
+    HWS_TRY(checkHip(g, hipGetLastError(), "occupancy launch"));
     hipLaunchKernelGGL(itemBinsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
                        kp.itemStart, pl.nbins, kp.itemBin);
     HWS_TRY(checkHip(g, hipGetLastError(), "item bin launch"));
@@ -1959,7 +1978,8 @@ GridArgs gridArgs(const hws_gridder* g, int pass) {
 }
 }  // namespace
 
-hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate) {
+hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate,
+                      bool sparseLayers) {
   Plan& pl = g->plan;
   if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
   GridArgs a = gridArgs(g, pass);
@@ -1982,9 +2002,10 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
       a.imgs.n = n;
       for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = (i0 + i < kp.nImg) ? kp.img[i0 + i] : 0;
       const size_t lds = gridLdsBytes(g, n, false);
-      if (!accumulate) {
+      if (!accumulate && !sparseLayers) {
         // the kernel writes the tiles of non-empty bins only: clear the pass's
-        // layers of these images first
+        // layers of these images first (unless the caller reads them through
+        // the occupancy bits)
         const size_t HW = size_t(g->W) * g->H;
         const size_t words = size_t(a.layerHi - a.layerLo) * HW * sizeof(float2) / 16;
         for (int i = 0; i < n; ++i) {
