@@ -328,14 +328,14 @@ def facet_leg(args, dev):
 
 
 def fft_traffic_from_profile():
-    """HBM bytes per 4096^2 layer transform (row + column launch) measured
-    with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the same
-    counters on a copy of known size; read from the committed summary
+    """HBM bytes per 4096^2 forward layer transform (row + sparse column
+    launch) measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by
+    the same counters on a copy of known size; read from the committed summary
     (profiles/fft_blocked_4096_traffic.md), or None when it is absent."""
     path = os.path.join(ROOT, "profiles", "fft_blocked_4096_traffic.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_layer_transform:"):
+            if line.startswith("traffic_bytes_per_forward_layer_transform:"):
                 return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
@@ -380,13 +380,29 @@ def main():
     log(f"{args.precision}: {ms_step * 1e3:.1f} ms/step, passes={npass}, phases(ms)={ph}")
     value = w.nvis / ms_step
 
-    # dominant kernel class: the per-layer 2-D FFT (row + column launches) of
-    # the forward (predict) direction, which reads and writes every element;
-    # the backward transform skips the uv tiles without samples
+    # dominant kernel class: the per-layer 2-D FFT of the forward (predict)
+    # direction (fftRowsBlockedKernel + fftColsBlockedSparseKernel). Its
+    # algorithmic bytes per layer transform: the row pass reads and writes
+    # every element (16 B), the column pass reads every element (8 B) and
+    # writes the uv tiles the degridding reads (8 B per cell of a needed
+    # tile), counted by the plan
     n_fft_2d = 4 * Workload.NW  # layer transforms per direction (4 images x 64 layers)
     fft_ms = ph["fft_pred"]
-    bytes_per_fft = 4 * Workload.W * Workload.H * 8  # 2 passes x (read + write) x c64
-    fft_gbs = bytes_per_fft * n_fft_2d / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else 0.0
+    HW = Workload.W * Workload.H
+    need = st_pred["tiles_needed"] if st_pred else [0, 0]
+    occ = st_pred["tiles_occupied"] if st_pred else [0, 0]
+    ntiles = st_pred["tiles_total"] if st_pred else 0
+    # images per kind in this workload: XX, YY real; XY, YX complex
+    fwd_bytes = 0.0
+    inv_bytes = 0.0
+    for k, nimg in ((0, 2), (1, 2)):
+        frac_need = need[k] / ntiles if ntiles else 1.0
+        frac_occ = occ[k] / ntiles if ntiles else 1.0
+        fwd_bytes += nimg * Workload.NW * HW * (24 + 8 * frac_need)
+        inv_bytes += nimg * Workload.NW * HW * (8 * frac_occ + 24)
+    bytes_per_fft = fwd_bytes / n_fft_2d
+    fft_gbs = fwd_bytes / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else 0.0
+    inv_gbs = inv_bytes / (ph["fft_inv"] * 1e-3) / 1e9 if ph["fft_inv"] > 0 else 0.0
     grid_ms = ph["grid"]
     gridded = st_inv["samples_gridded"] if st_inv else 0
     # 245 FP32 flops per gridded visibility (49 taps x 5); both image kinds
@@ -400,12 +416,15 @@ def main():
     roofline = {
         "bound": "hbm",
         "kernel": "forward layer 2-D FFT, blocked layout (fftRowsBlockedKernel + "
-                  "fftColsBlockedKernel)",
+                  "fftColsBlockedSparseKernel)",
         "achieved": round(fft_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(fft_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "algorithmic_bytes_per_layer_transform": bytes_per_fft,
+        "algorithmic_bytes_per_layer_transform": round(bytes_per_fft),
         "layer_transforms_per_direction": n_fft_2d,
         "avg_ms_per_layer_transform": round(fft_ms / n_fft_2d, 5) if n_fft_2d else None,
+        "tiles_needed_fraction": [round(n / ntiles, 4) if ntiles else None for n in need],
+        "backward_transform_gbs": round(inv_gbs, 1),
+        "tiles_occupied_fraction": [round(n / ntiles, 4) if ntiles else None for n in occ],
     }
 
     plain = None

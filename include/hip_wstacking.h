@@ -118,6 +118,13 @@ typedef struct hws_stats {
   uint64_t bins_used;         /* non-empty (layer, tile) bins */
   uint64_t chunks;            /* work items of the binned grid/degrid kernels */
   double ms_bin, ms_grid, ms_fft, ms_project, ms_total; /* device times */
+  /* Per image kind (0: real images, 1: complex images), over all layers of the
+   * plan: (layer, uv tile) bins holding records, and tiles a degridding
+   * subgrid reads. The sparse layer transforms read (backward) or write
+   * (forward) only these tiles in their first (second) pass. */
+  uint64_t tiles_total;
+  uint64_t tiles_occupied[2];
+  uint64_t tiles_needed[2];
 } hws_stats;
 
 /* Replaces: WStackingGridder<T>::WStackingGridder (gridding/wstackinggridder.h:120-122,

@@ -44,6 +44,8 @@ class HwsStats(C.Structure):
         ("bins_used", C.c_uint64), ("chunks", C.c_uint64),
         ("ms_bin", C.c_double), ("ms_grid", C.c_double), ("ms_fft", C.c_double),
         ("ms_project", C.c_double), ("ms_total", C.c_double),
+        ("tiles_total", C.c_uint64), ("tiles_occupied", C.c_uint64 * 2),
+        ("tiles_needed", C.c_uint64 * 2),
     ]
 
 

@@ -197,8 +197,10 @@ int layersInPass(const hws_gridder* g, int pass) {
   return passLayerStart(g, pass + 1) - passLayerStart(g, pass);
 }
 
-// sparse: the pass's layers hold only the tiles of bins with records (the
-// binned grid without the clear); the backward transform skips the others.
+// sparse, backward: the pass's layers hold only the tiles of bins with
+// records (the binned grid without the clear) and the transform skips the
+// others. sparse, forward: only the tiles the binned degridding reads are
+// written.
 hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false) {
   const int n = layersInPass(g, pass);
   const size_t HW = size_t(g->W) * g->H;
@@ -206,7 +208,8 @@ hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false
     FftSparse sp{};
     if (sparse) {
       const Plan& pl = g->plan;
-      sp.occ = pl.kinds[g->isComplex[img] ? 1 : 0].occ;
+      const KindPlan& kp = pl.kinds[g->isComplex[img] ? 1 : 0];
+      sp.occ = sign > 0 ? kp.occ : kp.need;
       sp.layer0 = passLayerStart(g, pass);
       sp.tilesY = pl.tilesY;
       sp.tshift = 0;
@@ -438,6 +441,7 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.itemStart);
     freeDev(kp.itemBin);
     freeDev(kp.occ);
+    freeDev(kp.need);
     freeDev(kp.partials);
     freeDev(kp.keys);
     freeDev(kp.keysAlt);
@@ -808,7 +812,8 @@ hws_status hws_predict_planned_device(hws_gridder* g, const float* d_real,
   for (int pass = 0; pass < g->nPasses; ++pass) {
     HWS_TRY(imageToLayers(g, pass));
     HWS_TRY(record(g, 3));
-    HWS_TRY(fftPassLayers(g, pass, -1));
+    HWS_TRY(fftPassLayers(g, pass, -1,
+                          g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && sparseInvert(g)));
     HWS_TRY(record(g, 4));
     if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32)
       HWS_TRY(degridPlain(g, pl.nrow, pl.nchan, pl.uvw, pl.wavelengths, d_vis_out, pass));
@@ -877,7 +882,7 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
     s.ms_total = total;
   }
   if (g->plan.counters) {
-    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long c[kPlanCounters] = {};
     HWS_TRY(checkHip(g, hipMemcpyAsync(c, g->plan.counters, sizeof(c), hipMemcpyDeviceToHost,
                                        g->stream), "counters"));
     HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "counters"));
@@ -886,6 +891,11 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
     s.samples_off_grid = c[1] + c[4];
     s.samples_off_layer = c[2] + c[5];
     s.bins_used = c[6] + c[7];
+    s.tiles_total = g->plan.nbins;
+    for (int k = 0; k < 2; ++k) {
+      s.tiles_occupied[k] = c[8 + k];
+      s.tiles_needed[k] = c[10 + k];
+    }
     for (const auto& kp : g->plan.kinds)
       if (kp.active) s.chunks += kp.nWorkItems;
   }

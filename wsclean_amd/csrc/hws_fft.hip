@@ -283,9 +283,11 @@ __device__ __forceinline__ size_t blockedAddr(int pos, int f, int W) {
   return (size_t(pos >> 2) * W + f) * 4 + (pos & 3);
 }
 
-// live (rows pass only, may be null): one flag per uv tile of the row block;
-// cells of a tile without samples read as zero (their memory is not read,
-// and need not have been cleared). tshift = log2 of the tile edge.
+// live (may be null), one flag per uv tile of the block's row block (rows
+// pass) or column block (columns pass); tshift = log2 of the tile edge. Rows
+// pass: cells of a tile without samples read as zero (their memory is not
+// read, and need not have been cleared). Columns pass: the cells of a tile
+// that is not live are not written.
 template <int SIGN, int R, int N, int NS, bool ROWS>
 __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g, int W,
                                              const float2* __restrict__ tw, int tid,
@@ -326,8 +328,11 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int pos = base + r * NS;
-      if (LAST)
+      if (LAST) {
+        // columns pass of a forward transform: tiles nobody reads are not written
+        if (!ROWS && live && !live[pos >> tshift]) continue;
         g[blockedAddr<ROWS>(pos, f, W)] = v[q][r];
+      }
       else
         lds[padIdx(pos * 4 + f)] = v[q][r];
     }
@@ -380,6 +385,31 @@ fftRowsBlockedSparseKernel(float2* __restrict__ data, int H, const float2* __res
   __syncthreads();
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
   blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x, live, tshift);
+}
+
+// Columns pass of a forward transform whose output is read only in some uv
+// tiles (the degridding subgrids): need holds one bit per (layer, tile) bin
+// as occ does for fftRowsBlockedSparseKernel; other tiles are not written.
+template <int SIGN, int N>
+__global__ void __launch_bounds__(N / 4)
+fftColsBlockedSparseKernel(float2* __restrict__ data, int W, const float2* __restrict__ tw,
+                           const uint32_t* __restrict__ need, int layer0, int tilesY,
+                           int tshift) {
+  __shared__ float2 lds[4 * N + (4 * N) / 64];
+  __shared__ unsigned char live[N / 4];
+  const int nb = gridDim.x;
+  int cb = blockIdx.x;
+  if ((nb & 7) == 0) cb = (cb & 7) * (nb >> 3) + (cb >> 3);
+  const int tilesX = W >> tshift;
+  const int tx = (cb * 4) >> tshift;
+  const size_t layerBase = size_t(layer0 + int(blockIdx.y)) * tilesY;
+  for (int t = threadIdx.x; t < tilesY; t += blockDim.x) {
+    const size_t b = (layerBase + t) * tilesX + tx;
+    live[t] = (need[b >> 5] >> (b & 31)) & 1u;
+  }
+  __syncthreads();
+  float2* g = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + size_t(cb) * 16;
+  blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x, live, tshift);
 }
 
 // grid (W / 4, nbatch): block = columns 4*cb .. 4*cb+3 of layer blockIdx.y;
@@ -645,6 +675,26 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
         default: break;
       }
       ok = ok && launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);
+    } else if (sign < 0 && sparse) {
+      ok = launchBlocked<-1>(W, true, data, W, H, nbatch, twRow, g->stream);
+      bool cols = false;
+      switch (H) {
+#define HWS_SPARSE_COLS(NN)                                                                    \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL((fftColsBlockedSparseKernel<-1, NN>), dim3(W / 4, nbatch), dim3(NN / 4), \
+                       0, g->stream, data, W, twCol, sparse->occ, sparse->layer0,              \
+                       sparse->tilesY, sparse->tshift);                                        \
+    cols = true;                                                                               \
+    break;
+        HWS_SPARSE_COLS(256)
+        HWS_SPARSE_COLS(512)
+        HWS_SPARSE_COLS(1024)
+        HWS_SPARSE_COLS(2048)
+        HWS_SPARSE_COLS(4096)
+#undef HWS_SPARSE_COLS
+        default: break;
+      }
+      ok = ok && cols;
     } else if (sign > 0)
       ok = launchBlocked<+1>(W, true, data, W, H, nbatch, twRow, g->stream) &&
            launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);

@@ -16,6 +16,7 @@
 namespace hws {
 
 constexpr int kMaxImages = 4;
+constexpr int kPlanCounters = 12;
 constexpr int kMaxFftStages = 24;
 constexpr int kMaxKernelSize = 15;
 constexpr int kMaxOversampling = 1023;
@@ -85,6 +86,10 @@ struct KindPlan {
   size_t itemBinCapacity = 0;
   uint32_t* occ = nullptr;  // bit per bin: the bin holds records [(nbins + 31) / 32]
   size_t occCapacity = 0;
+  // bit per bin: a degridding subgrid reads the tile (the bin or its left,
+  // upper or upper-left neighbour holds records)
+  uint32_t* need = nullptr;
+  size_t needCapacity = 0;
   float2* partials = nullptr;      // [nExtra][nImg][T * T]
   uint64_t nExtra = 0;
   size_t extraCapacity = 0, partialCapacity = 0;
@@ -105,6 +110,9 @@ struct Plan {
   uint32_t* scanScratch = nullptr;
   size_t scanCapacity = 0, scanScratchCapacity = 0;
   // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}], [6 + kind]: bins used
+  // [0..2] gridded / off grid / off layer (real kind), [3..5] the same
+  // (complex kind), [6 + kind] bins used, [8 + kind] occupied tiles,
+  // [10 + kind] needed tiles
   unsigned long long* counters = nullptr;
   uint32_t* queue = nullptr;  // work-queue head of the running grid launch
   void* sortTemp = nullptr;  // radix sort scratch
@@ -214,9 +222,11 @@ hws_status imageToLayers(hws_gridder* g, int pass);
 hws_status makeFftPlan(hws_gridder* g, int n, FftPlan& plan);
 void freeFftPlan(FftPlan& plan);
 // Unnormalised 2-D transform of nbatch H x W complex arrays, in place.
-// Uv tiles without samples in the layers of a backward (inverse) transform:
-// their memory is skipped (not read, need not be cleared). Blocked layout
-// only; see fftRowsBlockedSparseKernel.
+// Sparse layer transforms (blocked layout only). Backward: uv tiles without
+// samples are skipped on input (not read, need not be cleared; occ = the
+// occupancy bits). Forward: only the tiles a degridding subgrid reads are
+// written on output (occ = the need bits). See fftRowsBlockedSparseKernel and
+// fftColsBlockedSparseKernel.
 struct FftSparse {
   const uint32_t* occ;  // one bit per (layer, tile) bin
   int layer0;           // layer of the first transform of the batch

@@ -367,6 +367,45 @@ occupancyKernel(const uint32_t* __restrict__ binStart, size_t nbins,
   occ[w] = bits;
 }
 
+// Adds the set bits of a bit array to *out (thread per 32-bit word).
+__global__ void __launch_bounds__(kThreads)
+popcountKernel(const uint32_t* __restrict__ bits, size_t words,
+               unsigned long long* __restrict__ out) {
+  const size_t w = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int c = w < words ? __popc(bits[w]) : 0;
+  // wave sum, one atomic per wave
+  int v = c;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, (unsigned long long)v);
+}
+
+// Need bits: tile (tx, ty) of a layer is read by the degridding subgrids of
+// bins (tx, ty), (tx - 1, ty), (tx, ty - 1) and (tx - 1, ty - 1) (the
+// (T + K - 1)^2 subgrid reaches K - 1 <= T cells into the next tiles,
+// wrapping round the grid). Thread per 32-bit word.
+__global__ void __launch_bounds__(kThreads)
+needKernel(const uint32_t* __restrict__ occ, size_t nbins, int tilesX, int tilesY,
+           uint32_t* __restrict__ need) {
+  const size_t w = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t b0 = w * 32;
+  if (b0 >= nbins) return;
+  const size_t perLayer = size_t(tilesX) * tilesY;
+  auto bit = [&](size_t b) { return (occ[b >> 5] >> (b & 31)) & 1u; };
+  uint32_t bits = 0;
+  for (int j = 0; j < 32 && b0 + j < nbins; ++j) {
+    const size_t b = b0 + j;
+    const size_t layerBase = (b / perLayer) * perLayer;
+    const int t = int(b - layerBase);
+    const int ty = t / tilesX, tx = t - ty * tilesX;
+    const int px = tx == 0 ? tilesX - 1 : tx - 1, py = ty == 0 ? tilesY - 1 : ty - 1;
+    const uint32_t v = bit(b) | bit(layerBase + size_t(ty) * tilesX + px) |
+                       bit(layerBase + size_t(py) * tilesX + tx) |
+                       bit(layerBase + size_t(py) * tilesX + px);
+    bits |= v << j;
+  }
+  need[w] = bits;
+}
+
 // One work item of a pass: bin, item index within the bin (0 = the item
 // that writes the tile) and its record range. The items of the pass's bins
 // [binLo, binHi) are numbered in bin order from itemStart[binLo]; the bin of
@@ -1858,9 +1897,10 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
   HWS_TRY(ensureAlloc(g, pl.scanScratch, pl.scanScratchCapacity, scanScratchSize(pl.nbins),
                       "plan scan scratch"));
   if (!pl.counters)
-    HWS_TRY(checkHip(g, hipMalloc(&pl.counters, 8 * sizeof(unsigned long long)), "plan counters"));
+    HWS_TRY(checkHip(g, hipMalloc(&pl.counters, kPlanCounters * sizeof(unsigned long long)),
+                     "plan counters"));
   if (!pl.queue) HWS_TRY(checkHip(g, hipMalloc(&pl.queue, 64 * sizeof(uint32_t)), "plan queue"));
-  HWS_TRY(launchFill(g, pl.counters, 0u, 16));
+  HWS_TRY(launchFill(g, pl.counters, 0u, 2 * kPlanCounters));
 
   const GeomArgs ga = geomArgs(g);
   pl.chunk = workChunk(g, n);
@@ -1917,6 +1957,15 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     hipLaunchKernelGGL(occupancyKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
                        kp.binStart, pl.nbins, kp.occ);
     HWS_TRY(checkHip(g, hipGetLastError(), "occupancy launch"));
+    HWS_TRY(ensureAlloc(g, kp.need, kp.needCapacity, occWords, "plan need bits"));
+    hipLaunchKernelGGL(needKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
+                       kp.occ, pl.nbins, pl.tilesX, pl.tilesY, kp.need);
+    HWS_TRY(checkHip(g, hipGetLastError(), "need launch"));
+    hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
+                       kp.occ, occWords, pl.counters + 8 + kind);
+    hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
+                       kp.need, occWords, pl.counters + 10 + kind);
+    HWS_TRY(checkHip(g, hipGetLastError(), "tile count launch"));
     hipLaunchKernelGGL(itemBinsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
                        kp.itemStart, pl.nbins, kp.itemBin);
     HWS_TRY(checkHip(g, hipGetLastError(), "item bin launch"));

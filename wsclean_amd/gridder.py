@@ -250,7 +250,11 @@ class DeviceGridder(_Context):
     def stats(self):
         s = HwsStats()
         self._check(self.L.hws_get_stats(self.h, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in HwsStats._fields_}
+        out = {}
+        for k, _ in HwsStats._fields_:
+            v = getattr(s, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
 
 
 class Graph:
