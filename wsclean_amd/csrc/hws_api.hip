@@ -201,10 +201,26 @@ int layersInPass(const hws_gridder* g, int pass) {
 // records (the binned grid without the clear) and the transform skips the
 // others. sparse, forward: only the tiles the binned degridding reads are
 // written.
-hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false) {
+// fromModel (forward, with sparse): the row pass computes the layers from the
+// model images (imageToLayers is not run).
+hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false,
+                         bool fromModel = false) {
   const int n = layersInPass(g, pass);
   const size_t HW = size_t(g->W) * g->H;
   for (int img = 0; img < g->nImg; ++img) {
+    ModelSource ms{};
+    if (fromModel) {
+      ms.re = g->dModelRe + size_t(img) * HW;
+      ms.im = g->dModelIm ? g->dModelIm + size_t(img) * HW : nullptr;
+      ms.cplx = g->isComplex[img] ? 1 : 0;
+      ms.precision = g->cfg.precision;
+      ms.dl = g->cfg.pixel_size_x;
+      ms.dm = g->cfg.pixel_size_y;
+      ms.lShift = g->cfg.l_shift;
+      ms.mShift = g->cfg.m_shift;
+      ms.map = layerMap(g, ms.cplx);
+      ms.layer0 = passLayerStart(g, pass);
+    }
     FftSparse sp{};
     if (sparse) {
       const Plan& pl = g->plan;
@@ -216,7 +232,7 @@ hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false
       while ((1 << sp.tshift) < pl.tileSize) ++sp.tshift;
     }
     HWS_TRY(fft2d(g, g->dLayers + size_t(img) * g->maxLayersPerPass * HW, n, sign,
-                  sparse ? &sp : nullptr));
+                  sparse ? &sp : nullptr, fromModel ? &ms : nullptr));
   }
   return HWS_OK;
 }
@@ -810,10 +826,12 @@ hws_status hws_predict_planned_device(hws_gridder* g, const float* d_real,
   HWS_TRY(launchFill(g, d_vis_out, 0x7FC00000u, pl.nrow * pl.nchan * size_t(g->nImg) * 2));
   float ms[3] = {0, 0, 0};
   for (int pass = 0; pass < g->nPasses; ++pass) {
-    HWS_TRY(imageToLayers(g, pass));
+    // blocked binned form: the forward row pass computes the layers from the
+    // model and the column pass writes only the tiles degridding reads
+    const bool sparse = g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && sparseInvert(g);
+    if (!sparse) HWS_TRY(imageToLayers(g, pass));
     HWS_TRY(record(g, 3));
-    HWS_TRY(fftPassLayers(g, pass, -1,
-                          g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && sparseInvert(g)));
+    HWS_TRY(fftPassLayers(g, pass, -1, sparse, sparse));
     HWS_TRY(record(g, 4));
     if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32)
       HWS_TRY(degridPlain(g, pl.nrow, pl.nchan, pl.uvw, pl.wavelengths, d_vis_out, pass));

@@ -288,10 +288,44 @@ __device__ __forceinline__ size_t blockedAddr(int pos, int f, int W) {
 // pass: cells of a tile without samples read as zero (their memory is not
 // read, and need not have been cleared). Columns pass: the cells of a tile
 // that is not live are not written.
-template <int SIGN, int R, int N, int NS, bool ROWS>
+// The layer cell (x, y) of a forward transform computed from the model image
+// (the arithmetic of imageToLayersKernel, so the results are identical).
+__device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int x, int y, int W,
+                                           int H) {
+  int yDest = y + H / 2;
+  if (yDest >= H) yDest -= H;
+  int xDest = (W - x) + W / 2;
+  if (xDest >= W) xDest -= W;
+  const double m = ((double)yDest - (H / 2)) * ms.dm + ms.mShift;
+  const double l = ((W / 2) - (double)xDest) * ms.dl + ms.lShift;
+  double np = nPrime(l, m);
+  if (ms.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
+  const size_t src = size_t(yDest) * W + xDest;
+  const float re = ms.re[src];
+  const float imv = ms.cplx ? -ms.im[src] : 0.0f;
+  if (ms.precision == HWS_PRECISION_REFERENCE) {
+    double sn, cs;
+    sincos(2.0 * 3.14159265358979323846 * w * np, &sn, &cs);
+    if (ms.cplx) return make_float2(float(re * cs + imv * sn), float(imv * cs - re * sn));
+    return make_float2(float(re * cs), float(-re * sn));
+  }
+  const double phi = w * np;
+  const float f = float(phi - rint(phi));
+  float sn, cs;
+  sincospif(2.0f * f, &sn, &cs);
+  if (ms.cplx) return make_float2(re * cs + imv * sn, imv * cs - re * sn);
+  return make_float2(re * cs, -re * sn);
+}
+
+// MODEL (rows pass of a forward transform): the first stage computes its
+// inputs from the model image (ms, w; y0 = the row block's first row)
+// instead of reading the layer.
+template <int SIGN, int R, int N, int NS, bool ROWS, bool MODEL = false>
 __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g, int W,
                                              const float2* __restrict__ tw, int tid,
-                                             const unsigned char* live, int tshift) {
+                                             const unsigned char* live, int tshift,
+                                             const ModelSource* ms = nullptr, double w = 0.0,
+                                             int y0 = 0, int H = 0) {
   constexpr int B = N / R;
   constexpr int T = N / 4;
   constexpr int PER = 16 / R;
@@ -308,7 +342,9 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
     for (int r = 0; r < R; ++r) {
       const int pos = b + r * B;
       float2 x;
-      if (FIRST)
+      if (FIRST && MODEL)
+        x = modelCell(*ms, w, pos, y0 + f, W, H);
+      else if (FIRST)
         x = (ROWS && live && !live[pos >> tshift]) ? make_float2(0.0f, 0.0f)
                                                     : g[blockedAddr<ROWS>(pos, f, W)];
       else
@@ -340,17 +376,19 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
   if (!LAST) __syncthreads();
 }
 
-template <int SIGN, int N, bool ROWS, int NS = 1>
+template <int SIGN, int N, bool ROWS, int NS = 1, bool MODEL = false>
 __device__ __forceinline__ void blockedFft(float2* lds, float2* __restrict__ g, int W,
                                            const float2* __restrict__ tw, int tid,
                                            const unsigned char* live = nullptr,
-                                           int tshift = 0) {
+                                           int tshift = 0, const ModelSource* ms = nullptr,
+                                           double w = 0.0, int y0 = 0, int H = 0) {
   if constexpr (NS < N) {
     constexpr int REM = N / NS;
     constexpr int R = (REM % 16 == 0) ? 16 : REM;  // 16s, then the remaining 8, 4 or 2
     static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix");
-    blockedStage<SIGN, R, N, NS, ROWS>(lds, g, W, tw, tid, live, tshift);
-    blockedFft<SIGN, N, ROWS, NS * R>(lds, g, W, tw, tid, live, tshift);
+    blockedStage<SIGN, R, N, NS, ROWS, MODEL && NS == 1>(lds, g, W, tw, tid, live, tshift, ms,
+                                                         w, y0, H);
+    blockedFft<SIGN, N, ROWS, NS * R, MODEL>(lds, g, W, tw, tid, live, tshift, ms, w, y0, H);
   }
 }
 
@@ -361,6 +399,20 @@ fftRowsBlockedKernel(float2* __restrict__ data, int H, const float2* __restrict_
   __shared__ float2 lds[4 * N + (4 * N) / 64];
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
   blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x);
+}
+
+// Rows pass of a forward transform whose layers are computed from the model
+// image on the fly (the layer memory is written by this pass only): grid
+// (H / 4, nbatch), batch entry b = layer ms.layer0 + b.
+template <int SIGN, int N>
+__global__ void __launch_bounds__(N / 4)
+fftRowsFromModelKernel(float2* __restrict__ data, int H, const float2* __restrict__ tw,
+                       ModelSource ms) {
+  __shared__ float2 lds[4 * N + (4 * N) / 64];
+  float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
+  const double w = layerToW(ms.map, ms.layer0 + int(blockIdx.y));
+  blockedFft<SIGN, N, true, 1, true>(lds, g, N, tw, threadIdx.x, nullptr, 0, &ms, w,
+                                     int(blockIdx.x) * 4, H);
 }
 
 // The same over layers whose uv tiles without samples were never written:
@@ -649,7 +701,8 @@ bool blockedLayoutSupported(int W, int H) {
   return ok(W) && ok(H);
 }
 
-hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSparse* sparse) {
+hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSparse* sparse,
+                 const ModelSource* model) {
   if (nbatch <= 0) return HWS_OK;
   const int W = g->W, H = g->H;
   if (g->blocked) {
@@ -676,7 +729,26 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
       }
       ok = ok && launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);
     } else if (sign < 0 && sparse) {
-      ok = launchBlocked<-1>(W, true, data, W, H, nbatch, twRow, g->stream);
+      if (model) {
+        ok = false;
+        switch (W) {
+#define HWS_MODEL_ROWS(NN)                                                                      \
+  case NN:                                                                                      \
+    hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN>), dim3(H / 4, nbatch), dim3(NN / 4), 0,  \
+                       g->stream, data, H, twRow, *model);                                      \
+    ok = true;                                                                                  \
+    break;
+          HWS_MODEL_ROWS(256)
+          HWS_MODEL_ROWS(512)
+          HWS_MODEL_ROWS(1024)
+          HWS_MODEL_ROWS(2048)
+          HWS_MODEL_ROWS(4096)
+#undef HWS_MODEL_ROWS
+          default: break;
+        }
+      } else {
+        ok = launchBlocked<-1>(W, true, data, W, H, nbatch, twRow, g->stream);
+      }
       bool cols = false;
       switch (H) {
 #define HWS_SPARSE_COLS(NN)                                                                    \

@@ -51,6 +51,12 @@ struct LayerMap {
   int isComplex;
 };
 
+// n' - 1 at (l, m) (0 outside the unit circle), as the reference's sqrt
+// lookup tables hold it (gridding/wstackinggridder.cpp:851-872, 909-932).
+__host__ __device__ inline double nPrime(double l, double m) {
+  return (l * l + m * m < 1.0) ? sqrt(1.0 - l * l - m * m) - 1.0 : 0.0;
+}
+
 __host__ __device__ inline double layerToW(const LayerMap& m, int layer) {
   if (m.nW == 1) return 0.0;
   if (m.isComplex) return layer * (m.maxW + m.maxW) / (m.nW - 1) - m.maxW;
@@ -233,8 +239,20 @@ struct FftSparse {
   int tilesY;
   int tshift;           // log2 of the tile edge
 };
+// The model image a forward transform's row pass reads instead of the
+// layers: each layer cell is computed on the fly as imageToLayersKernel
+// writes it (copyImageToLayerAndInverseCorrect, wstackinggridder.cpp:934-969).
+struct ModelSource {
+  const float* re;  // this image's model (remapped and corrected), row-major
+  const float* im;  // imaginary part (complex images) or null
+  int cplx;
+  int precision;
+  double dl, dm, lShift, mShift;
+  LayerMap map;
+  int layer0;  // absolute layer of the first transform of the batch
+};
 hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign,
-                 const FftSparse* sparse = nullptr);
+                 const FftSparse* sparse = nullptr, const ModelSource* model = nullptr);
 
 // Storage index of layer cell (x, y): natural row-major, or blocked with
 // groups of four rows interleaved element by element.

@@ -1478,10 +1478,6 @@ struct ImgArgs {
 
 // n' = sqrt(1 - l^2 - m^2) - 1 inside the horizon, else 0
 // (gridding/wstackinggridder.cpp:865-868, 925-928).
-__device__ __forceinline__ double nPrime(double l, double m) {
-  return (l * l + m * m < 1.0) ? sqrt(1.0 - l * l - m * m) - 1.0 : 0.0;
-}
-
 __device__ __forceinline__ void kahanAdd(float& sum, float& comp, float term) {
   const float y = term - comp;
   const float t = sum + y;
