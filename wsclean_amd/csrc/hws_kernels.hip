@@ -79,6 +79,7 @@ struct GeomArgs {
   int W, H, K, OS, mid;
   double dl, dm;
   int tileSize, tilesX, tilesY;
+  int tileShift;     // log2(tileSize) when it is a power of two, else -1
   int rotate;        // apply the l/m shift phase rotation to samples
   double lShift, mShift;
   int blocked;       // layer storage layout (cellIndex)
@@ -213,13 +214,13 @@ struct TileSpan {
   int xoA, xoB;
 };
 
-__device__ __forceinline__ TileSpan tileSpan(int xs, int K, int T, int W) {
+__device__ __forceinline__ TileSpan tileSpan(int xs, int K, int T, int tshift, int W) {
   TileSpan s;
-  s.tA = xs / T;
+  s.tA = tshift >= 0 ? (xs >> tshift) : xs / T;
   s.xoA = xs - s.tA * T;
   const int last = xs + K - 1;
   const int lastW = last >= W ? last - W : last;
-  const int tl = lastW / T;
+  const int tl = tshift >= 0 ? (lastW >> tshift) : lastW / T;
   if (tl == s.tA && last < W) {
     s.tB = -1;
     s.xoB = 0;
@@ -230,15 +231,22 @@ __device__ __forceinline__ TileSpan tileSpan(int xs, int K, int T, int W) {
   return s;
 }
 
-// Bin and record of emission e (0: (A, A), 1: (B, A), 2: (A, B), 3: (B, B))
-// of a placed sample; returns false if that tile combination does not exist.
-__device__ __forceinline__ bool emission(const GeomArgs& ga, const Placement& p, int e,
-                                         uint32_t idx, uint32_t& bin, uint2& rec) {
+// The tile spans of a placed sample's footprint along x and y.
+__device__ __forceinline__ void footprintSpans(const GeomArgs& ga, const Placement& p,
+                                               TileSpan& sx, TileSpan& sy) {
   int xs = p.x - ga.mid, ys = p.y - ga.mid;
   if (xs < 0) xs += ga.W;
   if (ys < 0) ys += ga.H;
-  const TileSpan sx = tileSpan(xs, ga.K, ga.tileSize, ga.W);
-  const TileSpan sy = tileSpan(ys, ga.K, ga.tileSize, ga.H);
+  sx = tileSpan(xs, ga.K, ga.tileSize, ga.tileShift, ga.W);
+  sy = tileSpan(ys, ga.K, ga.tileSize, ga.tileShift, ga.H);
+}
+
+// Bin and record of emission e (0: (A, A), 1: (B, A), 2: (A, B), 3: (B, B))
+// of a placed sample with footprint spans sx, sy; returns false if that tile
+// combination does not exist.
+__device__ __forceinline__ bool emission(const GeomArgs& ga, const Placement& p,
+                                         const TileSpan& sx, const TileSpan& sy, int e,
+                                         uint32_t idx, uint32_t& bin, uint2& rec) {
   const bool useBx = e & 1, useBy = e & 2;
   if ((useBx && sx.tB < 0) || (useBy && sy.tB < 0)) return false;
   const int tx = useBx ? sx.tB : sx.tA, xo = useBx ? sx.xoB : sx.xoA;
@@ -291,21 +299,41 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t 
       waveCount(&sCount[1], i < n && p.layer >= 0 && !p.onGrid);
       waveCount(&sCount[2], i < n && p.layer < 0);
     }
+    TileSpan sx, sy;
+    sx.tA = sx.tB = sy.tA = sy.tB = -1;
+    sx.xoA = sx.xoB = sy.xoA = sy.xoB = 0;
+    if (ok) footprintSpans(ga, p, sx, sy);
+    // the four emissions first, then their runs, then all atomics at once
+    // (the scatter's returning atomics are in flight together)
+    uint32_t bin[4];
+    uint2 rec[4];
+    bool has[4];
+    LaneRun run[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      uint32_t bin = 0xFFFFFFFFu;
-      uint2 rec = make_uint2(0u, 0u);
-      const bool has = ok && emission(ga, p, e, uint32_t(i), bin, rec);
-      const LaneRun r = laneRun(bin, has);
-      if (!SCATTER) {
-        if (has && r.head == lane) atomicAdd(&k.counts[bin], uint32_t(r.len));
-      } else {
-        uint32_t pos = 0;
-        if (has && r.head == lane) pos = atomicAdd(&k.cursor[bin], uint32_t(r.len));
-        pos = __shfl(pos, r.head);
-        if (has) {
-          k.recSorted[pos + uint32_t(lane - r.head)] = rec;
-          if (k.sortKeys) k.sortKeys[pos + uint32_t(lane - r.head)] = (bin << 12) | (rec.y & 4095u);
+      bin[e] = 0xFFFFFFFFu;
+      rec[e] = make_uint2(0u, 0u);
+      has[e] = ok && emission(ga, p, sx, sy, e, uint32_t(i), bin[e], rec[e]);
+      run[e] = laneRun(bin[e], has[e]);
+    }
+    if (!SCATTER) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (has[e] && run[e].head == lane) atomicAdd(&k.counts[bin[e]], uint32_t(run[e].len));
+    } else {
+      uint32_t pos[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        pos[e] = 0;
+        if (has[e] && run[e].head == lane)
+          pos[e] = atomicAdd(&k.cursor[bin[e]], uint32_t(run[e].len));
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t at = __shfl(pos[e], run[e].head) + uint32_t(lane - run[e].head);
+        if (has[e]) {
+          k.recSorted[at] = rec[e];
+          if (k.sortKeys) k.sortKeys[at] = (bin[e] << 12) | (rec[e].y & 4095u);
         }
       }
     }
@@ -1778,6 +1806,9 @@ GeomArgs geomArgs(const hws_gridder* g) {
   ga.tileSize = g->plan.tileSize;
   ga.tilesX = g->plan.tilesX;
   ga.tilesY = g->plan.tilesY;
+  ga.tileShift = -1;
+  for (int sh = 0; sh < 8; ++sh)
+    if ((1 << sh) == ga.tileSize) ga.tileShift = sh;
   ga.rotate = g->cfg.rotate_visibilities &&
               (g->cfg.l_shift != 0.0 || g->cfg.m_shift != 0.0);
   ga.lShift = g->cfg.l_shift;
