@@ -1154,21 +1154,39 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
 #pragma unroll
       for (int gi = 0; gi < NIMG; ++gi) myVis[lane * NIMG + gi] = v[gi];
       waveSync();
-      const int nb = int(min(64u, count - b0));
-      for (int u0 = 0; u0 < nb; u0 += 4) {
-        const int r = u0 + lk;
-        const int ps = myPos[r];
-        // B[k][x = li] = xKernel, A[y = li][k] = v * yKernel
-        // (yKernelValue * xKernel[i], gridding/wstackinggridder.cpp:574)
-        const float bx = kt[(ps & 0xFFFF) + li];
-        const float ky = kt[(ps >> 16) + li];
+      // all 64 staged records, sixteen (four groups of four) per round with
+      // their LDS reads issued before the first product; the staging is zero
+      // past the item's end, and a fully unrolled batch keeps the
+      // accumulators in place (a partial last batch costs at most 63 zero
+      // records)
 #pragma unroll
-        for (int gi = 0; gi < NIMG; ++gi) {
-          const float2 vv = myVis[r * NIMG + gi];
-          acc[2 * gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(vv.x * ky, bx, acc[2 * gi], 0, 0, 0);
-          acc[2 * gi + 1] =
-              __builtin_amdgcn_mfma_f32_16x16x4f32(vv.y * ky, bx, acc[2 * gi + 1], 0, 0, 0);
+      for (int u0 = 0; u0 < 64; u0 += 16) {
+        int ps[4];
+        float2 vv[4][NIMG];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = u0 + 4 * q + lk;
+          ps[q] = myPos[r];
+#pragma unroll
+          for (int gi = 0; gi < NIMG; ++gi) vv[q][gi] = myVis[r * NIMG + gi];
         }
+        float bx[4], ky[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          // B[k][x = li] = xKernel, A[y = li][k] = v * yKernel
+          // (yKernelValue * xKernel[i], gridding/wstackinggridder.cpp:574)
+          bx[q] = kt[(ps[q] & 0xFFFF) + li];
+          ky[q] = kt[(ps[q] >> 16) + li];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int gi = 0; gi < NIMG; ++gi) {
+            acc[2 * gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(vv[q][gi].x * ky[q], bx[q],
+                                                               acc[2 * gi], 0, 0, 0);
+            acc[2 * gi + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(vv[q][gi].y * ky[q], bx[q],
+                                                                   acc[2 * gi + 1], 0, 0, 0);
+          }
       }
       waveSync();  // the staging is rewritten next batch
     }
