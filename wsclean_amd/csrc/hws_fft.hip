@@ -430,12 +430,19 @@ fftRowsBlockedSparseKernel(float2* __restrict__ data, int H, const float2* __res
   const int tilesX = N >> tshift;
   const int ty = (int(blockIdx.x) * 4) >> tshift;
   const size_t base = (size_t(layer0 + int(blockIdx.y)) * tilesY + ty) * tilesX;
+  int any = 0;
   for (int t = threadIdx.x; t < tilesX; t += blockDim.x) {
     const size_t b = base + t;
     live[t] = (occ[b >> 5] >> (b & 31)) & 1u;
+    any |= live[t];
   }
-  __syncthreads();
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
+  if (!__syncthreads_or(any)) {
+    // no tile of the row block holds samples: its transform is zero
+    float4* g4 = reinterpret_cast<float4*>(g);
+    for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x, live, tshift);
 }
 
@@ -455,11 +462,14 @@ fftColsBlockedSparseKernel(float2* __restrict__ data, int W, const float2* __res
   const int tilesX = W >> tshift;
   const int tx = (cb * 4) >> tshift;
   const size_t layerBase = size_t(layer0 + int(blockIdx.y)) * tilesY;
+  int any = 0;
   for (int t = threadIdx.x; t < tilesY; t += blockDim.x) {
     const size_t b = (layerBase + t) * tilesX + tx;
     live[t] = (need[b >> 5] >> (b & 31)) & 1u;
+    any |= live[t];
   }
-  __syncthreads();
+  // no tile of the column block is read afterwards: nothing to do
+  if (!__syncthreads_or(any)) return;
   float2* g = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + size_t(cb) * 16;
   blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x, live, tshift);
 }
