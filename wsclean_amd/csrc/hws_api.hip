@@ -226,6 +226,7 @@ hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false
       const Plan& pl = g->plan;
       const KindPlan& kp = pl.kinds[g->isComplex[img] ? 1 : 0];
       sp.occ = sign > 0 ? kp.occ : kp.need;
+      sp.rowLive = kp.rowLive;
       sp.layer0 = passLayerStart(g, pass);
       sp.tilesY = pl.tilesY;
       sp.tshift = 0;
@@ -458,6 +459,7 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.itemBin);
     freeDev(kp.occ);
     freeDev(kp.need);
+    freeDev(kp.rowLive);
     freeDev(kp.partials);
     freeDev(kp.keys);
     freeDev(kp.keysAlt);

@@ -283,11 +283,11 @@ __device__ __forceinline__ size_t blockedAddr(int pos, int f, int W) {
   return (size_t(pos >> 2) * W + f) * 4 + (pos & 3);
 }
 
-// live (may be null), one flag per uv tile of the block's row block (rows
-// pass) or column block (columns pass); tshift = log2 of the tile edge. Rows
-// pass: cells of a tile without samples read as zero (their memory is not
-// read, and need not have been cleared). Columns pass: the cells of a tile
-// that is not live are not written.
+// live (may be null): one flag per tile-sized stretch of the block's rows
+// (rows pass: per uv tile along x) or columns (columns pass: per tile row
+// along y), indexed by pos >> tshift (tshift = log2 of the tile edge).
+// liveIn: cells of a stretch that is not live read as zero (their memory is
+// not read, and need not have been written); otherwise they are not written.
 // The layer cell (x, y) of a forward transform computed from the model image
 // (the arithmetic of imageToLayersKernel, so the results are identical).
 __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int x, int y, int W,
@@ -324,6 +324,7 @@ template <int SIGN, int R, int N, int NS, bool ROWS, bool MODEL = false>
 __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g, int W,
                                              const float2* __restrict__ tw, int tid,
                                              const unsigned char* live, int tshift,
+                                             bool liveIn,
                                              const ModelSource* ms = nullptr, double w = 0.0,
                                              int y0 = 0, int H = 0) {
   constexpr int B = N / R;
@@ -345,7 +346,7 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
       if (FIRST && MODEL)
         x = modelCell(*ms, w, pos, y0 + f, W, H);
       else if (FIRST)
-        x = (ROWS && live && !live[pos >> tshift]) ? make_float2(0.0f, 0.0f)
+        x = (live && liveIn && !live[pos >> tshift]) ? make_float2(0.0f, 0.0f)
                                                     : g[blockedAddr<ROWS>(pos, f, W)];
       else
         x = lds[padIdx(pos * 4 + f)];
@@ -366,7 +367,7 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
       const int pos = base + r * NS;
       if (LAST) {
         // columns pass of a forward transform: tiles nobody reads are not written
-        if (!ROWS && live && !live[pos >> tshift]) continue;
+        if (live && !liveIn && !live[pos >> tshift]) continue;
         g[blockedAddr<ROWS>(pos, f, W)] = v[q][r];
       }
       else
@@ -380,15 +381,17 @@ template <int SIGN, int N, bool ROWS, int NS = 1, bool MODEL = false>
 __device__ __forceinline__ void blockedFft(float2* lds, float2* __restrict__ g, int W,
                                            const float2* __restrict__ tw, int tid,
                                            const unsigned char* live = nullptr,
-                                           int tshift = 0, const ModelSource* ms = nullptr,
+                                           int tshift = 0, bool liveIn = true,
+                                           const ModelSource* ms = nullptr,
                                            double w = 0.0, int y0 = 0, int H = 0) {
   if constexpr (NS < N) {
     constexpr int REM = N / NS;
     constexpr int R = (REM % 16 == 0) ? 16 : REM;  // 16s, then the remaining 8, 4 or 2
     static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix");
-    blockedStage<SIGN, R, N, NS, ROWS, MODEL && NS == 1>(lds, g, W, tw, tid, live, tshift, ms,
-                                                         w, y0, H);
-    blockedFft<SIGN, N, ROWS, NS * R, MODEL>(lds, g, W, tw, tid, live, tshift, ms, w, y0, H);
+    blockedStage<SIGN, R, N, NS, ROWS, MODEL && NS == 1>(lds, g, W, tw, tid, live, tshift,
+                                                         liveIn, ms, w, y0, H);
+    blockedFft<SIGN, N, ROWS, NS * R, MODEL>(lds, g, W, tw, tid, live, tshift, liveIn, ms, w, y0,
+                                             H);
   }
 }
 
@@ -411,7 +414,7 @@ fftRowsFromModelKernel(float2* __restrict__ data, int H, const float2* __restric
   __shared__ float2 lds[4 * N + (4 * N) / 64];
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
   const double w = layerToW(ms.map, ms.layer0 + int(blockIdx.y));
-  blockedFft<SIGN, N, true, 1, true>(lds, g, N, tw, threadIdx.x, nullptr, 0, &ms, w,
+  blockedFft<SIGN, N, true, 1, true>(lds, g, N, tw, threadIdx.x, nullptr, 0, true, &ms, w,
                                      int(blockIdx.x) * 4, H);
 }
 
@@ -437,13 +440,33 @@ fftRowsBlockedSparseKernel(float2* __restrict__ data, int H, const float2* __res
     any |= live[t];
   }
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
-  if (!__syncthreads_or(any)) {
-    // no tile of the row block holds samples: its transform is zero
-    float4* g4 = reinterpret_cast<float4*>(g);
-    for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
+  // no tile of the row block holds samples: its transform is zero, which the
+  // columns pass knows from the same bits (it is not written)
+  if (!__syncthreads_or(any)) return;
+  blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x, live, tshift, true);
+}
+
+// Columns pass of a backward transform after fftRowsBlockedSparseKernel:
+// rows of a tile row without samples were not written and read as zero.
+// rowLive holds one bit per (layer, tile row) (bit = layer * tilesY + ty).
+template <int SIGN, int N>
+__global__ void __launch_bounds__(N / 4)
+fftColsBlockedRowSparseKernel(float2* __restrict__ data, int W, const float2* __restrict__ tw,
+                              const uint32_t* __restrict__ rowLive, int layer0, int tilesY,
+                              int tshift) {
+  __shared__ float2 lds[4 * N + (4 * N) / 64];
+  __shared__ unsigned char live[N / 4];
+  const int nb = gridDim.x;
+  int cb = blockIdx.x;
+  if ((nb & 7) == 0) cb = (cb & 7) * (nb >> 3) + (cb >> 3);
+  const size_t base = size_t(layer0 + int(blockIdx.y)) * tilesY;
+  for (int t = threadIdx.x; t < tilesY; t += blockDim.x) {
+    const size_t b = base + t;
+    live[t] = (rowLive[b >> 5] >> (b & 31)) & 1u;
   }
-  blockedFft<SIGN, N, true>(lds, g, N, tw, threadIdx.x, live, tshift);
+  __syncthreads();
+  float2* g = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + size_t(cb) * 16;
+  blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x, live, tshift, true);
 }
 
 // Columns pass of a forward transform whose output is read only in some uv
@@ -471,7 +494,7 @@ fftColsBlockedSparseKernel(float2* __restrict__ data, int W, const float2* __res
   // no tile of the column block is read afterwards: nothing to do
   if (!__syncthreads_or(any)) return;
   float2* g = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + size_t(cb) * 16;
-  blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x, live, tshift);
+  blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x, live, tshift, false);
 }
 
 // grid (W / 4, nbatch): block = columns 4*cb .. 4*cb+3 of layer blockIdx.y;
@@ -737,7 +760,24 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
 #undef HWS_SPARSE_CASE
         default: break;
       }
-      ok = ok && launchBlocked<+1>(H, false, data, W, H, nbatch, twCol, g->stream);
+      bool cols = false;
+      switch (H) {
+#define HWS_ROWSPARSE_COLS(NN)                                                                 \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL((fftColsBlockedRowSparseKernel<+1, NN>), dim3(W / 4, nbatch),            \
+                       dim3(NN / 4), 0, g->stream, data, W, twCol, sparse->rowLive,            \
+                       sparse->layer0, sparse->tilesY, sparse->tshift);                        \
+    cols = true;                                                                               \
+    break;
+        HWS_ROWSPARSE_COLS(256)
+        HWS_ROWSPARSE_COLS(512)
+        HWS_ROWSPARSE_COLS(1024)
+        HWS_ROWSPARSE_COLS(2048)
+        HWS_ROWSPARSE_COLS(4096)
+#undef HWS_ROWSPARSE_COLS
+        default: break;
+      }
+      ok = ok && cols;
     } else if (sign < 0 && sparse) {
       if (model) {
         ok = false;

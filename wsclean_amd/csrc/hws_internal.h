@@ -96,6 +96,9 @@ struct KindPlan {
   // upper or upper-left neighbour holds records)
   uint32_t* need = nullptr;
   size_t needCapacity = 0;
+  // bit per (layer, tile row): some tile of the row holds records
+  uint32_t* rowLive = nullptr;
+  size_t rowLiveCapacity = 0;
   float2* partials = nullptr;      // [nExtra][nImg][T * T]
   uint64_t nExtra = 0;
   size_t extraCapacity = 0, partialCapacity = 0;
@@ -234,6 +237,7 @@ void freeFftPlan(FftPlan& plan);
 // written on output (occ = the need bits). See fftRowsBlockedSparseKernel and
 // fftColsBlockedSparseKernel.
 struct FftSparse {
+  const uint32_t* rowLive;  // backward: one bit per (layer, tile row) holding samples
   const uint32_t* occ;  // one bit per (layer, tile) bin
   int layer0;           // layer of the first transform of the batch
   int tilesY;

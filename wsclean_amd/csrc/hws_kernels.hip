@@ -395,6 +395,27 @@ occupancyKernel(const uint32_t* __restrict__ binStart, size_t nbins,
   occ[w] = bits;
 }
 
+// Row bits: bit (layer * tilesY + ty) is set when some tile of that tile row
+// holds records (thread per bit, packed by wave ballots).
+__global__ void __launch_bounds__(kThreads)
+rowLiveKernel(const uint32_t* __restrict__ occ, size_t nrows, int tilesX,
+              uint32_t* __restrict__ rowLive) {
+  const size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool live = false;
+  if (r < nrows) {
+    const size_t b0 = r * size_t(tilesX);
+    for (int t = 0; t < tilesX && !live; ++t) {
+      const size_t b = b0 + t;
+      live = (occ[b >> 5] >> (b & 31)) & 1u;
+    }
+  }
+  const unsigned long long m = __ballot(live);
+  const size_t w0 = (r & ~size_t(63)) >> 5;  // first word of this wave's 64 bits
+  const int lane = threadIdx.x & 63;
+  if (lane == 0 && w0 < (nrows + 31) / 32) rowLive[w0] = uint32_t(m);
+  if (lane == 32 && w0 + 1 < (nrows + 31) / 32) rowLive[w0 + 1] = uint32_t(m >> 32);
+}
+
 // Adds the set bits of a bit array to *out (thread per 32-bit word).
 __global__ void __launch_bounds__(kThreads)
 popcountKernel(const uint32_t* __restrict__ bits, size_t words,
@@ -2002,6 +2023,12 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     hipLaunchKernelGGL(occupancyKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
                        kp.binStart, pl.nbins, kp.occ);
     HWS_TRY(checkHip(g, hipGetLastError(), "occupancy launch"));
+    const size_t nrowsBits = pl.nbins / size_t(pl.tilesX);
+    const size_t rowWords = (nrowsBits + 31) / 32;
+    HWS_TRY(ensureAlloc(g, kp.rowLive, kp.rowLiveCapacity, rowWords, "plan row bits"));
+    hipLaunchKernelGGL(rowLiveKernel, dim3(blocksFor(nrowsBits)), dim3(kThreads), 0, g->stream,
+                       kp.occ, nrowsBits, pl.tilesX, kp.rowLive);
+    HWS_TRY(checkHip(g, hipGetLastError(), "row bits launch"));
     HWS_TRY(ensureAlloc(g, kp.need, kp.needCapacity, occWords, "plan need bits"));
     hipLaunchKernelGGL(needKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
                        kp.occ, pl.nbins, pl.tilesX, pl.tilesY, kp.need);
