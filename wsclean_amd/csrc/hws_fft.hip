@@ -289,7 +289,9 @@ __device__ __forceinline__ size_t blockedAddr(int pos, int f, int W) {
 // liveIn: cells of a stretch that is not live read as zero (their memory is
 // not read, and need not have been written); otherwise they are not written.
 // The layer cell (x, y) of a forward transform computed from the model image
-// (the arithmetic of imageToLayersKernel, so the results are identical).
+// (the arithmetic of imageToLayersKernel, so the results are identical). REF:
+// the reference precision (float n', FP64 sincos).
+template <bool REF>
 __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int x, int y, int W,
                                            int H) {
   int yDest = y + H / 2;
@@ -298,12 +300,12 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
   if (xDest >= W) xDest -= W;
   const double m = ((double)yDest - (H / 2)) * ms.dm + ms.mShift;
   const double l = ((W / 2) - (double)xDest) * ms.dl + ms.lShift;
-  double np = nPrime(l, m);
-  if (ms.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
+  double np = nPrimeSeries(l, m);
+  if (REF) np = double(float(np));
   const size_t src = size_t(yDest) * W + xDest;
   const float re = ms.re[src];
   const float imv = ms.cplx ? -ms.im[src] : 0.0f;
-  if (ms.precision == HWS_PRECISION_REFERENCE) {
+  if (REF) {
     double sn, cs;
     sincos(2.0 * 3.14159265358979323846 * w * np, &sn, &cs);
     if (ms.cplx) return make_float2(float(re * cs + imv * sn), float(imv * cs - re * sn));
@@ -320,7 +322,7 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
 // MODEL (rows pass of a forward transform): the first stage computes its
 // inputs from the model image (ms, w; y0 = the row block's first row)
 // instead of reading the layer.
-template <int SIGN, int R, int N, int NS, bool ROWS, bool MODEL = false>
+template <int SIGN, int R, int N, int NS, bool ROWS, int MODEL = 0>
 __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g, int W,
                                              const float2* __restrict__ tw, int tid,
                                              const unsigned char* live, int tshift,
@@ -344,7 +346,7 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
       const int pos = b + r * B;
       float2 x;
       if (FIRST && MODEL)
-        x = modelCell(*ms, w, pos, y0 + f, W, H);
+        x = modelCell<MODEL == 2>(*ms, w, pos, y0 + f, W, H);
       else if (FIRST)
         x = (live && liveIn && !live[pos >> tshift]) ? make_float2(0.0f, 0.0f)
                                                     : g[blockedAddr<ROWS>(pos, f, W)];
@@ -377,7 +379,7 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
   if (!LAST) __syncthreads();
 }
 
-template <int SIGN, int N, bool ROWS, int NS = 1, bool MODEL = false>
+template <int SIGN, int N, bool ROWS, int NS = 1, int MODEL = 0>
 __device__ __forceinline__ void blockedFft(float2* lds, float2* __restrict__ g, int W,
                                            const float2* __restrict__ tw, int tid,
                                            const unsigned char* live = nullptr,
@@ -388,7 +390,7 @@ __device__ __forceinline__ void blockedFft(float2* lds, float2* __restrict__ g, 
     constexpr int REM = N / NS;
     constexpr int R = (REM % 16 == 0) ? 16 : REM;  // 16s, then the remaining 8, 4 or 2
     static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix");
-    blockedStage<SIGN, R, N, NS, ROWS, MODEL && NS == 1>(lds, g, W, tw, tid, live, tshift,
+    blockedStage<SIGN, R, N, NS, ROWS, NS == 1 ? MODEL : 0>(lds, g, W, tw, tid, live, tshift,
                                                          liveIn, ms, w, y0, H);
     blockedFft<SIGN, N, ROWS, NS * R, MODEL>(lds, g, W, tw, tid, live, tshift, liveIn, ms, w, y0,
                                              H);
@@ -407,15 +409,15 @@ fftRowsBlockedKernel(float2* __restrict__ data, int H, const float2* __restrict_
 // Rows pass of a forward transform whose layers are computed from the model
 // image on the fly (the layer memory is written by this pass only): grid
 // (H / 4, nbatch), batch entry b = layer ms.layer0 + b.
-template <int SIGN, int N>
+template <int SIGN, int N, bool REF>
 __global__ void __launch_bounds__(N / 4)
 fftRowsFromModelKernel(float2* __restrict__ data, int H, const float2* __restrict__ tw,
                        ModelSource ms) {
   __shared__ float2 lds[4 * N + (4 * N) / 64];
   float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
   const double w = layerToW(ms.map, ms.layer0 + int(blockIdx.y));
-  blockedFft<SIGN, N, true, 1, true>(lds, g, N, tw, threadIdx.x, nullptr, 0, true, &ms, w,
-                                     int(blockIdx.x) * 4, H);
+  blockedFft<SIGN, N, true, 1, REF ? 2 : 1>(lds, g, N, tw, threadIdx.x, nullptr, 0, true, &ms, w,
+                                            int(blockIdx.x) * 4, H);
 }
 
 // The same over layers whose uv tiles without samples were never written:
@@ -784,8 +786,12 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
         switch (W) {
 #define HWS_MODEL_ROWS(NN)                                                                      \
   case NN:                                                                                      \
-    hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN>), dim3(H / 4, nbatch), dim3(NN / 4), 0,  \
-                       g->stream, data, H, twRow, *model);                                      \
+    if (model->precision == HWS_PRECISION_REFERENCE)                                            \
+      hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN, true>), dim3(H / 4, nbatch),           \
+                         dim3(NN / 4), 0, g->stream, data, H, twRow, *model);                   \
+    else                                                                                        \
+      hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN, false>), dim3(H / 4, nbatch),          \
+                         dim3(NN / 4), 0, g->stream, data, H, twRow, *model);                   \
     ok = true;                                                                                  \
     break;
           HWS_MODEL_ROWS(256)

@@ -1725,7 +1725,7 @@ imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
   if (xDest >= a.W) xDest -= a.W;
   const double m = ((double)yDest - (a.H / 2)) * a.dm + a.mShift;
   const double l = ((a.W / 2) - (double)xDest) * a.dl + a.lShift;
-  double np = nPrime(l, m);
+  double np = nPrimeSeries(l, m);  // as the forward transform's fused row pass
   if (a.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
   const size_t src = size_t(img) * HW + size_t(yDest) * a.W + xDest;
   const float re = modelRe[src];
