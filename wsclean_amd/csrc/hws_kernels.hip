@@ -81,6 +81,7 @@ struct GeomArgs {
   int tileSize, tilesX, tilesY;
   int tileShift;     // log2(tileSize) when it is a power of two, else -1
   int rotate;        // apply the l/m shift phase rotation to samples
+  int fastPhase;     // rotation phasor from an FP64-reduced FP32 sincospi
   double lShift, mShift;
   int blocked;       // layer storage layout (cellIndex)
 };
@@ -574,20 +575,32 @@ struct GridArgs {
   uint32_t chunk;    // records per work item
 };
 
-__device__ __forceinline__ float2 rotateSample(float2 v, const GridArgs& a,
-                                               const double* __restrict__ uvw,
-                                               const double* __restrict__ wl,
-                                               uint32_t idx, double sign) {
-  // MSGridderBase::rotateVisibilities (gridding/msgridderbase.cpp:692-704)
-  // with shiftFactor = sign * 2 pi (u l_shift + v m_shift), FP64 argument.
+// The phase-centre rotation of a sample as a unit phasor, once per sample
+// for all its images: MSGridderBase::rotateVisibilities
+// (gridding/msgridderbase.cpp:692-704) with shiftFactor = sign * 2 pi
+// (u l_shift + v m_shift) and rad = shiftFactor / lambda. The argument is
+// FP64; the mixed precision reduces it to a fraction of a turn in FP64 and
+// takes an FP32 sincospi, the reference precision an FP64 sincos.
+__device__ __forceinline__ float2 rotationPhasor(const GridArgs& a,
+                                                 const double* __restrict__ uvw,
+                                                 const double* __restrict__ wl, uint32_t idx,
+                                                 double sign) {
   const size_t row = idx / a.nchan, ch = idx - row * a.nchan;
-  const double shiftFactor = sign * 2.0 * kPi *
-                             (uvw[row * 3] * a.ga.lShift + uvw[row * 3 + 1] * a.ga.mShift);
-  const double rad = shiftFactor / wl[ch];
+  const double turns =
+      sign * (uvw[row * 3] * a.ga.lShift + uvw[row * 3 + 1] * a.ga.mShift) / wl[ch];
+  if (a.ga.fastPhase) {
+    const float f = float(turns - rint(turns));
+    float sf, cf;
+    sincospif(2.0f * f, &sf, &cf);
+    return make_float2(cf, sf);
+  }
   double s, c;
-  sincos(rad, &s, &c);
-  const float cf = float(c), sf = float(s);
-  return make_float2(v.x * cf - v.y * sf, v.x * sf + v.y * cf);
+  sincos(2.0 * kPi * turns, &s, &c);
+  return make_float2(float(c), float(s));
+}
+
+__device__ __forceinline__ float2 rotateBy(float2 v, float2 p) {
+  return make_float2(v.x * p.x - v.y * p.y, v.x * p.y + v.y * p.x);
 }
 
 // Kernel tables into LDS: FP32 tables, or FP16 (hi, lo) pairs whose sum
@@ -709,10 +722,12 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
           const int kk = int(rec.y >> 12);
           const int ky = kk / OS;
           myPos[lane] = make_int2(int(rec.y & 4095u), (ky * K) | ((kk - ky * OS) * K) << 16);
+          const float2 ph = a.ga.rotate ? rotationPhasor(a, uvw, wl, idx, -1.0)
+                                        : make_float2(1.0f, 0.0f);
 #pragma unroll
           for (int gi = 0; gi < NIMG; ++gi) {
             float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
-            if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, -1.0);
+            if (a.ga.rotate) v = rotateBy(v, ph);
             if (conj) v.y = -v.y;
             myVis[lane * NIMG + gi] = v;
           }
@@ -965,10 +980,12 @@ gridCellsKernel(GridArgs a, const uint32_t* __restrict__ binStart,
           pos = xs | (ys << 6);
           kof = (ky * K) | (kx * K) << 16;
         }
+        float2 ph = make_float2(1.0f, 0.0f);
+        if (ROT) ph = rotationPhasor(a, uvw, wl, idx, -1.0);
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
           float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
-          if (ROT) x = rotateSample(x, a, uvw, wl, idx, -1.0);
+          if (ROT) x = rotateBy(x, ph);
           if (conj) x.y = -x.y;
           v[gi] = x;
         }
@@ -1163,10 +1180,12 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
         const int xs = int(rec.y & 63u), ys = int((rec.y >> 6) & 63u);
         pos = (kx * rowLen + (T - 1) + (K - 1) - xs) |
               ((ky * rowLen + (T - 1) + (K - 1) - ys) << 16);
+        float2 ph = make_float2(1.0f, 0.0f);
+        if (ROT) ph = rotationPhasor(a, uvw, wl, idx, -1.0);
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
           float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
-          if (ROT) x = rotateSample(x, a, uvw, wl, idx, -1.0);
+          if (ROT) x = rotateBy(x, ph);
           if (conj) x.y = -x.y;
           v[gi] = x;
         }
@@ -1326,11 +1345,13 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
           }
         }
       }
+      const float2 ph = a.ga.rotate ? rotationPhasor(a, uvw, wl, idx, +1.0)
+                                    : make_float2(1.0f, 0.0f);
 #pragma unroll
       for (int gi = 0; gi < NIMG; ++gi) {
         // folded samples return the sample, others its conjugate (:681-684)
         float2 v = conj ? acc[gi] : make_float2(acc[gi].x, -acc[gi].y);
-        if (a.ga.rotate) v = rotateSample(v, a, uvw, wl, idx, +1.0);
+        if (a.ga.rotate) v = rotateBy(v, ph);
         out[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]] = v;
       }
     }
@@ -1852,6 +1873,7 @@ GeomArgs geomArgs(const hws_gridder* g) {
               (g->cfg.l_shift != 0.0 || g->cfg.m_shift != 0.0);
   ga.lShift = g->cfg.l_shift;
   ga.mShift = g->cfg.m_shift;
+  ga.fastPhase = g->cfg.precision == HWS_PRECISION_MIXED;
   ga.blocked = g->blocked;
   return ga;
 }
