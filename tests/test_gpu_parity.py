@@ -252,15 +252,29 @@ def test_nan_contract_off_grid_and_other_pass(hws):
     assert np.isnan(v1[0]) and np.isfinite(v1[1]) and np.isnan(v1[2])
 
 
-@pytest.mark.parametrize("shape", [(96, 96), (80, 96)])
-def test_non_power_of_two_sizes(hws, oracle, shape):
+@pytest.mark.parametrize("shape,precision", [((96, 96), "reference"), ((80, 96), "reference"),
+                                             ((144, 112), "mixed"), ((90, 90), "plain")])
+def test_non_power_of_two_sizes(hws, oracle, shape, precision):
+    # multiples of 4: the blocked layout with the runtime-radix transforms;
+    # 90 x 90 (plain form only: no tile edge divides it) the row-major layout
     w, h = shape
     uvw, wl, vis, dl = case(7, min(w, h))
     uvw_l = lambda_samples(uvw, wl)
     maxw = np.abs(uvw_l[:, 2]).max()
     img_o, _ = oracle_invert(oracle, w, h, dl, dl, uvw_l, vis.reshape(-1), 6, 0.0, maxw)
-    re, _, _, _ = gpu_invert(hws, w, h, dl, dl, uvw, wl, vis, 6, 0.0, maxw, "reference")
-    assert rel(re[0], img_o) < TOL_IMG["reference"]
+    re, _, _, _ = gpu_invert(hws, w, h, dl, dl, uvw, wl, vis, 6, 0.0, maxw, precision)
+    assert rel(re[0], img_o) < TOL_IMG[precision]
+    # predict through the same transforms
+    model = np.random.default_rng(71).standard_normal((h, w)).astype(np.float32)
+    v_o = oracle_predict(oracle, w, h, dl, dl, uvw_l, model, None, 6, 0.0, maxw)
+    g = hws.WStackingGridder(w, h, dl, dl, 7, 63, precision=precision)
+    g.prepare_w_layers(6, 0.0, 0.0, maxw)
+    g.initialize_prediction(model)
+    g.start_prediction_pass(0)
+    v = g.sample_data(uvw, wl).reshape(-1)
+    ok = ~np.isnan(v_o)
+    assert np.array_equal(np.isnan(v), ~ok)
+    assert rel(v[ok], v_o[ok]) < TOL_IMG[precision]
 
 
 def test_device_forms_match_host_protocol(hws):

@@ -601,6 +601,47 @@ fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
   }
 }
 
+// Blocked layout at sizes without a compile-time kernel (any 7-smooth size
+// with W and H multiples of 4): one workgroup per 4 rows (row pass) or 4
+// columns (column pass), read and written as whole 128-byte segments of the
+// blocked layout, transformed in LDS ([4][n + 1]) by the mixed-radix stages.
+template <int SIGN>
+__global__ void __launch_bounds__(1024)
+fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
+                            unsigned long long radices, const float2* __restrict__ tw) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int pitch = W + 1;
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(blockIdx.x) * 4 * W;
+  for (int i = tid; i < 4 * W; i += nthr) lds[(i & 3) * pitch + (i >> 2)] = base[i];
+  __syncthreads();
+  fftLds<SIGN>(lds, pitch, 4, W, radices, tw, tid, nthr);
+  for (int i = tid; i < 4 * W; i += nthr) base[i] = lds[(i & 3) * pitch + (i >> 2)];
+}
+
+template <int SIGN>
+__global__ void __launch_bounds__(1024)
+fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlocks,
+                            unsigned long long radices, const float2* __restrict__ tw) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  int blk = blockIdx.x;
+  if ((nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
+  const int pitch = H + 1;
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(blk) * 16;
+  // element i: row block i >> 4, column (i >> 2) & 3, row in block i & 3
+  for (int i = tid; i < 4 * H; i += nthr) {
+    const int yb = i >> 4, c = (i >> 2) & 3, r = i & 3;
+    lds[c * pitch + yb * 4 + r] = base[size_t(yb) * W * 4 + (i & 15)];
+  }
+  __syncthreads();
+  fftLds<SIGN>(lds, pitch, 4, H, radices, tw, tid, nthr);
+  for (int i = tid; i < 4 * H; i += nthr) {
+    const int yb = i >> 4, c = (i >> 2) & 3, r = i & 3;
+    base[size_t(yb) * W * 4 + (i & 15)] = lds[c * pitch + yb * 4 + r];
+  }
+}
+
 unsigned long long packRadices(const FftPlan& plan) {
   unsigned long long r = 0;
   for (int i = plan.nstages - 1; i >= 0; --i) r = (r << 4) | unsigned(plan.radix[i]);
@@ -731,15 +772,47 @@ bool launchBlocked(int n, bool rows, float2* data, int W, int H, int nbatch,
 }
 }  // namespace
 
+// Sizes with compile-time blocked kernels (and their sparse variants).
+bool fastBlockedSize(int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0; }
+
+// The blocked layout is used whenever both sizes are multiples of 4 and four
+// rows or columns fit in 64 KiB of LDS (n <= 2044) for the runtime-radix
+// kernels.
 bool blockedLayoutSupported(int W, int H) {
-  auto ok = [](int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0; };
-  return ok(W) && ok(H);
+  if (fastBlockedSize(W) && fastBlockedSize(H)) return true;
+  return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && W <= 2044 && H <= 2044;
 }
 
 hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSparse* sparse,
                  const ModelSource* model) {
   if (nbatch <= 0) return HWS_OK;
   const int W = g->W, H = g->H;
+  if (g->blocked && !(fastBlockedSize(W) && fastBlockedSize(H))) {
+    // blocked layout, runtime radices
+    if (sparse || model)
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "sparse transform needs a power-of-two size");
+    const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
+    const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
+    const int tr = threadsFor(g->rowPlan, 4), tc = threadsFor(g->colPlan, 4);
+    if (tr == 0 || tc == 0)
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "transform does not fit one workgroup");
+    const size_t ldsR = size_t(4) * (W + 1) * sizeof(float2);
+    const size_t ldsC = size_t(4) * (H + 1) * sizeof(float2);
+    const unsigned long long radR = packRadices(g->rowPlan), radC = packRadices(g->colPlan);
+    const int nColBlocks = W / 4;
+    if (sign > 0) {
+      hipLaunchKernelGGL(fftRowsBlockedGenericKernel<+1>, dim3(H / 4, nbatch), dim3(tr), ldsR,
+                         g->stream, data, W, H, radR, twRow);
+      hipLaunchKernelGGL(fftColsBlockedGenericKernel<+1>, dim3(nColBlocks, nbatch), dim3(tc),
+                         ldsC, g->stream, data, W, H, nColBlocks, radC, twCol);
+    } else {
+      hipLaunchKernelGGL(fftRowsBlockedGenericKernel<-1>, dim3(H / 4, nbatch), dim3(tr), ldsR,
+                         g->stream, data, W, H, radR, twRow);
+      hipLaunchKernelGGL(fftColsBlockedGenericKernel<-1>, dim3(nColBlocks, nbatch), dim3(tc),
+                         ldsC, g->stream, data, W, H, nColBlocks, radC, twCol);
+    }
+    return checkHip(g, hipGetLastError(), "fft blocked generic launch");
+  }
   if (g->blocked) {
     const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
     const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;

@@ -297,6 +297,7 @@ __host__ __device__ inline void cellOf(size_t p, int W, bool blocked, int& x, in
 }
 
 bool blockedLayoutSupported(int W, int H);
+bool fastBlockedSize(int n);
 
 inline int passLayerStart(const hws_gridder* g, int pass) {
   return int((size_t(g->nW) * size_t(pass)) / size_t(g->nPasses));
