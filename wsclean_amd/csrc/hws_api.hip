@@ -389,7 +389,12 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   if ((st = checkHip(g, hipMemcpy(g->dCorrYf, cy.data(), cy.size() * sizeof(float), hipMemcpyHostToDevice), "correction upload")) != HWS_OK) return bail(st);
 
   // blocked layer layout where the coalesced radix-16 transforms cover the size
-  g->blocked = blockedLayoutSupported(g->W, g->H);
+  // blocked layer layout: always at the power-of-two sizes with compile-time
+  // transforms; at other sizes for the binned forms only (the plain form's
+  // per-tap global atomics touch fewer cache lines in row-major order)
+  g->blocked = blockedLayoutSupported(g->W, g->H) &&
+               ((fastBlockedSize(g->W) && fastBlockedSize(g->H)) ||
+                c->precision != HWS_PRECISION_PLAIN_FP32);
   {
     const char* e = std::getenv("HWS_NO_MFMA");
     g->noMfma = e && e[0] == '1';
