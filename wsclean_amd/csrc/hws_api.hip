@@ -242,8 +242,7 @@ hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false
 // can skip them (power-of-two tile edge of at least 4).
 bool sparseInvert(const hws_gridder* g) {
   const int T = g->plan.tileSize;
-  return g->blocked && fastBlockedSize(g->W) && fastBlockedSize(g->H) &&
-         g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && T >= 4 &&
+  return g->blocked && g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && T >= 4 &&
          (T & (T - 1)) == 0;
 }
 

@@ -605,39 +605,93 @@ fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
 // with W and H multiples of 4): one workgroup per 4 rows (row pass) or 4
 // columns (column pass), read and written as whole 128-byte segments of the
 // blocked layout, transformed in LDS ([4][n + 1]) by the mixed-radix stages.
-template <int SIGN>
+// The sparse modes mirror the compile-time kernels:
+//   rows MODE 1 (backward): only tiles with samples are read (bits: the
+//     occupancy of (layer, tile), sp.occ); a row block without any is
+//     skipped (not written); MODE 2 (forward): the input is computed from the
+//     model image (ms);
+//   columns MODE 1 (backward): tile rows without samples (sp.rowLive) read as
+//     zero; MODE 2 (forward): only the tiles degridding reads (sp.occ = the
+//     need bits) are written, and a column block without any is skipped.
+struct GenericSparseArgs {
+  const uint32_t* bits;
+  int layer0, tilesX, tilesY, tshift;
+};
+
+__device__ __forceinline__ bool bitAt(const uint32_t* bits, size_t b) {
+  return (bits[b >> 5] >> (b & 31)) & 1u;
+}
+
+template <int SIGN, int MODE, bool REF>
 __global__ void __launch_bounds__(1024)
 fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
-                            unsigned long long radices, const float2* __restrict__ tw) {
+                            unsigned long long radices, const float2* __restrict__ tw,
+                            GenericSparseArgs sp, ModelSource ms) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int pitch = W + 1;
+  const int layer = sp.layer0 + int(blockIdx.y);
+  const int ty = (int(blockIdx.x) * 4) >> sp.tshift;
+  const size_t rowBase = (size_t(layer) * sp.tilesY + ty) * sp.tilesX;
+  if (MODE == 1) {
+    int any = 0;
+    for (int t = tid; t < sp.tilesX; t += nthr) any |= bitAt(sp.bits, rowBase + t);
+    if (!__syncthreads_or(any)) return;
+  }
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(blockIdx.x) * 4 * W;
-  for (int i = tid; i < 4 * W; i += nthr) lds[(i & 3) * pitch + (i >> 2)] = base[i];
+  const double w = MODE == 2 ? layerToW(ms.map, ms.layer0 + int(blockIdx.y)) : 0.0;
+  for (int i = tid; i < 4 * W; i += nthr) {
+    const int x = i >> 2, f = i & 3;
+    float2 v;
+    if (MODE == 2)
+      v = modelCell<REF>(ms, w, x, int(blockIdx.x) * 4 + f, W, H);
+    else if (MODE == 1 && !bitAt(sp.bits, rowBase + (x >> sp.tshift)))
+      v = make_float2(0.0f, 0.0f);
+    else
+      v = base[i];
+    lds[f * pitch + x] = v;
+  }
   __syncthreads();
   fftLds<SIGN>(lds, pitch, 4, W, radices, tw, tid, nthr);
   for (int i = tid; i < 4 * W; i += nthr) base[i] = lds[(i & 3) * pitch + (i >> 2)];
 }
 
-template <int SIGN>
+template <int SIGN, int MODE>
 __global__ void __launch_bounds__(1024)
 fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlocks,
-                            unsigned long long radices, const float2* __restrict__ tw) {
+                            unsigned long long radices, const float2* __restrict__ tw,
+                            GenericSparseArgs sp) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   int blk = blockIdx.x;
   if ((nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
   const int pitch = H + 1;
+  const int layer = sp.layer0 + int(blockIdx.y);
+  const int tx = (blk * 4) >> sp.tshift;
+  // MODE 1: bit (layer, tile row); MODE 2: bit (layer, tile row, this tile column)
+  auto live = [&](int y) -> bool {
+    const int ty = y >> sp.tshift;
+    if (MODE == 1) return bitAt(sp.bits, size_t(layer) * sp.tilesY + ty);
+    return bitAt(sp.bits, (size_t(layer) * sp.tilesY + ty) * sp.tilesX + tx);
+  };
+  if (MODE == 2) {
+    int any = 0;
+    for (int t = tid; t < sp.tilesY; t += nthr) any |= live(t << sp.tshift);
+    if (!__syncthreads_or(any)) return;
+  }
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(blk) * 16;
   // element i: row block i >> 4, column (i >> 2) & 3, row in block i & 3
   for (int i = tid; i < 4 * H; i += nthr) {
     const int yb = i >> 4, c = (i >> 2) & 3, r = i & 3;
-    lds[c * pitch + yb * 4 + r] = base[size_t(yb) * W * 4 + (i & 15)];
+    float2 v = make_float2(0.0f, 0.0f);
+    if (MODE != 1 || live(yb * 4 + r)) v = base[size_t(yb) * W * 4 + (i & 15)];
+    lds[c * pitch + yb * 4 + r] = v;
   }
   __syncthreads();
   fftLds<SIGN>(lds, pitch, 4, H, radices, tw, tid, nthr);
   for (int i = tid; i < 4 * H; i += nthr) {
     const int yb = i >> 4, c = (i >> 2) & 3, r = i & 3;
+    if (MODE == 2 && !live(yb * 4 + r)) continue;
     base[size_t(yb) * W * 4 + (i & 15)] = lds[c * pitch + yb * 4 + r];
   }
 }
@@ -788,9 +842,8 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
   if (nbatch <= 0) return HWS_OK;
   const int W = g->W, H = g->H;
   if (g->blocked && !(fastBlockedSize(W) && fastBlockedSize(H))) {
-    // blocked layout, runtime radices
-    if (sparse || model)
-      return setLastError(g, HWS_ERR_UNSUPPORTED, "sparse transform needs a power-of-two size");
+    // blocked layout, runtime radices (sparse modes as for the compile-time
+    // kernels)
     const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
     const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
     const int tr = threadsFor(g->rowPlan, 4), tc = threadsFor(g->colPlan, 4);
@@ -800,16 +853,50 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     const size_t ldsC = size_t(4) * (H + 1) * sizeof(float2);
     const unsigned long long radR = packRadices(g->rowPlan), radC = packRadices(g->colPlan);
     const int nColBlocks = W / 4;
+    GenericSparseArgs rs{}, cs{};
+    ModelSource msrc{};
+    if (sparse) {
+      rs.layer0 = cs.layer0 = sparse->layer0;
+      rs.tilesY = cs.tilesY = sparse->tilesY;
+      rs.tshift = cs.tshift = sparse->tshift;
+      rs.tilesX = cs.tilesX = W >> sparse->tshift;
+      if (sign > 0) {
+        rs.bits = sparse->occ;
+        cs.bits = sparse->rowLive;
+      } else {
+        cs.bits = sparse->occ;  // the need bits
+      }
+    }
+    if (model) msrc = *model;
+    const dim3 gr(H / 4, nbatch), gc(nColBlocks, nbatch);
     if (sign > 0) {
-      hipLaunchKernelGGL(fftRowsBlockedGenericKernel<+1>, dim3(H / 4, nbatch), dim3(tr), ldsR,
-                         g->stream, data, W, H, radR, twRow);
-      hipLaunchKernelGGL(fftColsBlockedGenericKernel<+1>, dim3(nColBlocks, nbatch), dim3(tc),
-                         ldsC, g->stream, data, W, H, nColBlocks, radC, twCol);
+      if (sparse) {
+        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<+1, 1, false>), gr, dim3(tr), ldsR,
+                           g->stream, data, W, H, radR, twRow, rs, msrc);
+        hipLaunchKernelGGL((fftColsBlockedGenericKernel<+1, 1>), gc, dim3(tc), ldsC, g->stream,
+                           data, W, H, nColBlocks, radC, twCol, cs);
+      } else {
+        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<+1, 0, false>), gr, dim3(tr), ldsR,
+                           g->stream, data, W, H, radR, twRow, rs, msrc);
+        hipLaunchKernelGGL((fftColsBlockedGenericKernel<+1, 0>), gc, dim3(tc), ldsC, g->stream,
+                           data, W, H, nColBlocks, radC, twCol, cs);
+      }
     } else {
-      hipLaunchKernelGGL(fftRowsBlockedGenericKernel<-1>, dim3(H / 4, nbatch), dim3(tr), ldsR,
-                         g->stream, data, W, H, radR, twRow);
-      hipLaunchKernelGGL(fftColsBlockedGenericKernel<-1>, dim3(nColBlocks, nbatch), dim3(tc),
-                         ldsC, g->stream, data, W, H, nColBlocks, radC, twCol);
+      if (model && model->precision == HWS_PRECISION_REFERENCE)
+        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<-1, 2, true>), gr, dim3(tr), ldsR,
+                           g->stream, data, W, H, radR, twRow, rs, msrc);
+      else if (model)
+        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<-1, 2, false>), gr, dim3(tr), ldsR,
+                           g->stream, data, W, H, radR, twRow, rs, msrc);
+      else
+        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<-1, 0, false>), gr, dim3(tr), ldsR,
+                           g->stream, data, W, H, radR, twRow, rs, msrc);
+      if (sparse)
+        hipLaunchKernelGGL((fftColsBlockedGenericKernel<-1, 2>), gc, dim3(tc), ldsC, g->stream,
+                           data, W, H, nColBlocks, radC, twCol, cs);
+      else
+        hipLaunchKernelGGL((fftColsBlockedGenericKernel<-1, 0>), gc, dim3(tc), ldsC, g->stream,
+                           data, W, H, nColBlocks, radC, twCol, cs);
     }
     return checkHip(g, hipGetLastError(), "fft blocked generic launch");
   }
