@@ -1278,12 +1278,94 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
   }
 }
 
+// The visibilities of one primary record from the subgrid around its tile
+// (sub: the NIMG images of a cell interleaved, row pitch sg cells). KK: the
+// kernel size when it is fixed at compile time (0: a.ga.K). The taps are
+// summed rows outer, columns inner, as the runtime-K form sums them.
+template <int NIMG, int KK>
+__device__ __forceinline__ void degridRecord(const GridArgs& a, uint2 rec,
+                                             const float* __restrict__ kt,
+                                             const float2* __restrict__ sub, int sg,
+                                             const double* __restrict__ uvw,
+                                             const double* __restrict__ wl,
+                                             float2* __restrict__ out) {
+  const int K = KK ? KK : a.ga.K, OS = a.ga.OS;
+  const uint32_t idx = rec.x & 0x3FFFFFFFu;
+  const bool conj = (rec.x >> 31) != 0;
+  const int xo = int(rec.y & 63u) - (K - 1), yo = int((rec.y >> 6) & 63u) - (K - 1);
+  const int kk = int(rec.y >> 12);
+  const int ky = kk / OS, kx = kk - ky * OS;
+  float2 acc[NIMG];
+#pragma unroll
+  for (int gi = 0; gi < NIMG; ++gi) acc[gi] = make_float2(0.0f, 0.0f);
+  if constexpr (KK > 0) {
+    float xk[KK];
+#pragma unroll
+    for (int i = 0; i < KK; ++i) xk[i] = kt[kx * KK + i];
+    // one row of taps in flight at a time: unrolling the rows hoists every
+    // LDS read of the footprint and leaves one wave per SIMD
+#pragma nounroll
+    for (int j = 0; j < KK; ++j) {
+      const float yk = kt[ky * KK + j];
+      const float2* row = sub + size_t((yo + j) * sg + xo) * NIMG;
+#pragma unroll
+      for (int i = 0; i < KK; ++i) {
+        // xKernel[i] * yKernelValue (gridding/wstackinggridder.cpp:668)
+        const float kv = xk[i] * yk;
+        if constexpr (NIMG % 2 == 0) {
+#pragma unroll
+          for (int gp = 0; gp < NIMG / 2; ++gp) {
+            const float4 gv = reinterpret_cast<const float4*>(row + i * NIMG)[gp];
+            acc[2 * gp].x += gv.x * kv;
+            acc[2 * gp].y += gv.y * kv;
+            acc[2 * gp + 1].x += gv.z * kv;
+            acc[2 * gp + 1].y += gv.w * kv;
+          }
+        } else {
+#pragma unroll
+          for (int gi = 0; gi < NIMG; ++gi) {
+            const float2 gv = row[i * NIMG + gi];
+            acc[gi].x += gv.x * kv;
+            acc[gi].y += gv.y * kv;
+          }
+        }
+      }
+    }
+  } else {
+    for (int j = 0; j < K; ++j) {
+      const float yk = kt[ky * K + j];
+      const float2* row = sub + size_t((yo + j) * sg + xo) * NIMG;
+      for (int i = 0; i < K; ++i) {
+        const float kv = kt[kx * K + i] * yk;
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          const float2 gv = row[i * NIMG + gi];
+          acc[gi].x += gv.x * kv;
+          acc[gi].y += gv.y * kv;
+        }
+      }
+    }
+  }
+  const float2 ph = a.ga.rotate ? rotationPhasor(a, uvw, wl, idx, +1.0) : make_float2(1.0f, 0.0f);
+#pragma unroll
+  for (int gi = 0; gi < NIMG; ++gi) {
+    // folded samples return the sample, others its conjugate (:681-684)
+    float2 v = conj ? acc[gi] : make_float2(acc[gi].x, -acc[gi].y);
+    if (a.ga.rotate) v = rotateBy(v, ph);
+    out[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]] = v;
+  }
+}
+
+constexpr int kDegridQueue = 128;  // staged primary records per wave
+
 // Degridding of one image kind, one work item per workgroup at a time: the
 // (T + K - 1)^2 subgrid around the item's (layer, tile) is gathered from the
-// layer into LDS; each lane then sums the K x K taps of one primary record
-// (the record of the tile holding the footprint start) and writes its
-// visibilities.
-template <int NIMG>
+// layer into LDS. Only the primary record of a footprint (the record of the
+// tile holding the footprint start) is degridded, so each wave first
+// compacts the primary records of its share of the item into a queue in
+// LDS and degrids them 64 at a time, one record per lane: no lane idles on
+// the secondary records of split footprints.
+template <int NIMG, int KK>
 __global__ void __launch_bounds__(kThreads)
 degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
                   const uint32_t* __restrict__ itemStart, const uint32_t* __restrict__ itemBin,
@@ -1292,9 +1374,13 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
                   const double* __restrict__ wl, const float2* __restrict__ layers,
                   float2* __restrict__ out) {
   extern __shared__ float smem[];
-  const int K = a.ga.K, OS = a.ga.OS, sg = a.sg, sg2 = a.sg * a.sg;
+  const int K = KK ? KK : a.ga.K, OS = a.ga.OS, sg = a.sg, sg2 = a.sg * a.sg;
   float* kt = smem;
-  float2* sub = reinterpret_cast<float2*>(smem + ((OS * K + 3) & ~3));  // [NIMG][sg2]
+  float2* sub = reinterpret_cast<float2*>(smem + ((OS * K + 3) & ~3));  // [sg2][NIMG]
+  uint2* queue =
+      reinterpret_cast<uint2*>(sub + size_t(NIMG) * sg2) + (threadIdx.x >> 6) * kDegridQueue;
+  const int lane = int(threadIdx.x & 63u);
+  const unsigned long long below = (1ull << lane) - 1ull;
   loadKernelTable(kt, ktab, khalf, OS * K);
   const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
@@ -1309,52 +1395,45 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
     const int x0 = tx * a.ga.tileSize, y0 = ty * a.ga.tileSize;
-    __syncthreads();  // previous bin's readers are done with sub
+    __syncthreads();  // previous item's readers are done with sub and the queues
     for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x) {
       const int cy = cell / sg, cx = cell - cy * sg;
       const int gx = (x0 + cx) % a.ga.W, gy = (y0 + cy) % a.ga.H;
 #pragma unroll
       for (int gi = 0; gi < NIMG; ++gi)
-        sub[gi * sg2 + cell] =
+        sub[cell * NIMG + gi] =
             layers[(size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW +
                    cellIndex(gx, gy, a.ga.W, a.ga.blocked)];
     }
     __syncthreads();
-    for (uint32_t it = threadIdx.x; it < count; it += blockDim.x) {
-      const uint2 rec = recs[start + it];
-      if (((rec.x >> 30) & 1u) == 0u) continue;  // secondary record of a split footprint
-      const uint32_t idx = rec.x & 0x3FFFFFFFu;
-      const bool conj = (rec.x >> 31) != 0;
-      const int xo = int(rec.y & 63u) - (K - 1), yo = int((rec.y >> 6) & 63u) - (K - 1);
-      const int kk = int(rec.y >> 12);
-      const int ky = kk / OS, kx = kk - ky * OS;
-      float2 acc[NIMG];
-#pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi) acc[gi] = make_float2(0.0f, 0.0f);
-      for (int j = 0; j < K; ++j) {
-        const float yk = kt[ky * K + j];
-        const int rowBase = (yo + j) * sg + xo;
-        for (int i = 0; i < K; ++i) {
-          // xKernel[i] * yKernelValue (gridding/wstackinggridder.cpp:668)
-          const float kv = kt[kx * K + i] * yk;
-#pragma unroll
-          for (int gi = 0; gi < NIMG; ++gi) {
-            const float2 gv = sub[gi * sg2 + rowBase + i];
-            acc[gi].x += gv.x * kv;
-            acc[gi].y += gv.y * kv;
-          }
-        }
+    // wave w reads records w * 64 + lane + k * blockDim; qn (wave-uniform)
+    // primary records wait in the queue, at most 63 between batches
+    int qn = 0;
+    for (uint32_t b = threadIdx.x & ~63u; b < count; b += blockDim.x) {
+      const uint32_t it = b + uint32_t(lane);
+      uint2 rec = make_uint2(0u, 0u);
+      bool primary = false;
+      if (it < count) {
+        rec = recs[start + it];
+        primary = ((rec.x >> 30) & 1u) != 0u;
       }
-      const float2 ph = a.ga.rotate ? rotationPhasor(a, uvw, wl, idx, +1.0)
-                                    : make_float2(1.0f, 0.0f);
-#pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi) {
-        // folded samples return the sample, others its conjugate (:681-684)
-        float2 v = conj ? acc[gi] : make_float2(acc[gi].x, -acc[gi].y);
-        if (a.ga.rotate) v = rotateBy(v, ph);
-        out[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]] = v;
+      const unsigned long long mask = __ballot(primary);
+      if (primary) queue[qn + __popcll(mask & below)] = rec;
+      qn += __popcll(mask);
+      if (qn >= 64) {
+        waveSync();
+        const uint2 mine = queue[lane];
+        const int left = qn - 64;
+        uint2 tail = make_uint2(0u, 0u);
+        if (lane < left) tail = queue[64 + lane];
+        waveSync();
+        if (lane < left) queue[lane] = tail;
+        qn = left;
+        degridRecord<NIMG, KK>(a, mine, kt, sub, sg, uvw, wl, out);
       }
     }
+    waveSync();
+    if (lane < qn) degridRecord<NIMG, KK>(a, queue[lane], kt, sub, sg, uvw, wl, out);
   }
 }
 
@@ -1907,7 +1986,8 @@ size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
   const size_t kt = size_t((g->OS * g->K + 3) & ~3) * sizeof(float);
   if (degrid) {
     const int sg = T + g->K - 1;
-    return kt + size_t(nimg) * sg * sg * sizeof(float2);
+    return kt + size_t(nimg) * sg * sg * sizeof(float2) +
+           size_t(kThreads / 64) * kDegridQueue * sizeof(uint2);
   }
   if (mfmaKernelUsable(g)) {
     const size_t table = size_t((g->OS * paddedRow(g->K, T) + 3) & ~3) * sizeof(float);
@@ -2304,12 +2384,18 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
     const size_t lds = gridLdsBytes(g, kp.nImg, true);
     float2* out = reinterpret_cast<float2*>(dVisOut);
     switch (kp.nImg) {
-#define HWS_DEGRID_CASE(N)                                                                 \
-  case N:                                                                                  \
-    hipLaunchKernelGGL(degridTilesKernel<N>, dim3(blocks), dim3(kThreads), lds, g->stream,  \
-                       a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted, g->dKernel,    \
-                       khalf, pl.uvw,                                                      \
-                       pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out);  \
+#define HWS_DEGRID_CASE(N)                                                                  \
+  case N:                                                                                   \
+    if (g->K == 7)                                                                          \
+      hipLaunchKernelGGL((degridTilesKernel<N, 7>), dim3(blocks), dim3(kThreads), lds,        \
+                         g->stream, a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted,   \
+                         g->dKernel, khalf, pl.uvw, pl.wavelengths,                         \
+                         reinterpret_cast<const float2*>(g->dLayers), out);                 \
+    else                                                                                    \
+      hipLaunchKernelGGL((degridTilesKernel<N, 0>), dim3(blocks), dim3(kThreads), lds,        \
+                         g->stream, a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted,   \
+                         g->dKernel, khalf, pl.uvw, pl.wavelengths,                         \
+                         reinterpret_cast<const float2*>(g->dLayers), out);                 \
     break;
       HWS_DEGRID_CASE(1)
       HWS_DEGRID_CASE(2)
