@@ -455,6 +455,10 @@ void hws_destroy(hws_gridder* g) {
   freeDev(pl.scanScratch);
   freeDev(pl.counters);
   freeDev(pl.queue);
+  freeDev(pl.rowKeys);
+  freeDev(pl.rowKeysAlt);
+  freeDev(pl.rowOrder);
+  freeDev(pl.rowOrderAlt);
   for (auto& kp : pl.kinds) {
     freeDev(kp.recSorted);
     freeDev(kp.counts);

@@ -16,7 +16,7 @@
 namespace hws {
 
 constexpr int kMaxImages = 4;
-constexpr int kPlanCounters = 12;
+constexpr int kPlanCounters = 14;
 constexpr int kMaxFftStages = 24;
 constexpr int kMaxKernelSize = 15;
 constexpr int kMaxOversampling = 1023;
@@ -139,9 +139,17 @@ struct Plan {
   // [kind * 3 + {0: gridded, 1: off grid, 2: off layer}], [6 + kind]: bins used
   // [0..2] gridded / off grid / off layer (real kind), [3..5] the same
   // (complex kind), [6 + kind] bins used, [8 + kind] occupied tiles,
-  // [10 + kind] needed tiles
+  // [10 + kind] needed tiles, [12 + kind] bins failing the plan check
   unsigned long long* counters = nullptr;
   uint32_t* queue = nullptr;  // work-queue head of the running grid launch
+  // rows sorted by their middle channel's bin: sort keys and row indices
+  // (and the second buffers of the sort)
+  uint32_t* rowKeys = nullptr;
+  uint32_t* rowKeysAlt = nullptr;
+  uint32_t* rowOrder = nullptr;
+  uint32_t* rowOrderAlt = nullptr;
+  size_t rowKeysCapacity = 0, rowKeysAltCapacity = 0, rowOrderCapacity = 0,
+         rowOrderAltCapacity = 0;
   void* sortTemp = nullptr;  // radix sort scratch
   size_t sortTempCapacity = 0;
 };
@@ -232,6 +240,8 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass);
 // buffers may have been swapped).
 hws_status sortRecords(hws_gridder* g, uint32_t*& keys, uint32_t*& keysAlt, uint2*& recs,
                        uint2*& recsAlt, size_t n, int endBit);
+hws_status sortRows(hws_gridder* g, uint32_t*& keys, uint32_t*& keysAlt, uint32_t*& rows,
+                    uint32_t*& rowsAlt, size_t n);
 hws_status gridPlain(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl, const float* dVis,
                      int pass);

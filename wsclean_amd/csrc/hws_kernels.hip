@@ -202,6 +202,7 @@ struct KindArgs {
   uint2* recSorted;              // scatter: records ordered by bin
   unsigned long long* counters;  // gridded, off grid, off layer
   uint32_t* sortKeys;            // scatter: bin << 12 | footprint origin, or null
+  uint32_t nRecords;             // scatter: records of the plan
 };
 
 // The uv tiles a sample's footprint touches. Footprint cells run from
@@ -270,73 +271,183 @@ __device__ __forceinline__ Placement placeSample(const GeomArgs& ga, const Layer
                uvw[row * 3 + 2] / lambda);
 }
 
-// Pass 1: place every sample (FP64, AddDataSample arithmetic) and histogram
-// the (layer, tile) bins of all tiles its footprint touches. Sample
-// i = row * nchan + channel, the order of the visibility array; the lanes of a
-// run of consecutive samples in one bin add their count with one atomic.
-// Pass 2 (SCATTER) repeats the placement and writes the records to their
-// bins. Workgroups stride over the samples; the gridded / off-grid /
-// off-layer counters are summed per workgroup and added to memory once
-// (a same-address atomic per wave serialises in L2).
-template <bool SCATTER>
+// Sort key of a row: the bin of the first tile its middle channel's footprint
+// touches (0xFFFFFFFF when that sample is not gridded). Rows with equal keys
+// have nearly equal uvw, so their samples fall in the same bins channel by
+// channel.
 __global__ void __launch_bounds__(kThreads)
-binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
-          int nchan, GeomArgs ga, KindArgs k) {
+rowKeyKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
+             int nchan, GeomArgs ga, LayerMap lm, uint32_t* __restrict__ keys,
+             uint32_t* __restrict__ rows) {
+  const size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  const double lambda = wl[nchan / 2];
+  const Placement p = place(ga, lm, uvw[r * 3] / lambda, uvw[r * 3 + 1] / lambda,
+                            uvw[r * 3 + 2] / lambda);
+  uint32_t key = 0xFFFFFFFFu;
+  if (p.layer >= 0 && p.onGrid) {
+    TileSpan sx, sy;
+    footprintSpans(ga, p, sx, sy);
+    uint2 rec;
+    emission(ga, p, sx, sy, 0, 0u, key, rec);
+  }
+  keys[r] = key;
+  rows[r] = uint32_t(r);
+}
+
+constexpr int kGroupSamples = 1024;  // samples of one row group (4 per thread)
+constexpr int kGroupPer = kGroupSamples / kThreads;
+constexpr int kBinSlots = 1024;      // LDS table of one group's bins
+constexpr uint32_t kNoBin = 0xFFFFFFFFu;
+
+// Slot of `bin` in a workgroup's LDS table, inserted if absent (linear
+// probing from a multiplicative hash); -1 when 16 probes find neither the bin
+// nor a free slot (the caller then counts in memory directly).
+__device__ __forceinline__ int binSlot(uint32_t* keys, uint32_t bin) {
+  uint32_t h = (bin * 2654435761u) >> 22;
+  for (int probe = 0; probe < 16; ++probe) {
+    const uint32_t k = *reinterpret_cast<volatile uint32_t*>(&keys[h]);
+    if (k == bin) return int(h);
+    if (k == kNoBin) {
+      const uint32_t old = atomicCAS(&keys[h], kNoBin, bin);
+      if (old == kNoBin || old == bin) return int(h);
+    }
+    h = (h + 1u) & uint32_t(kBinSlots - 1);
+  }
+  return -1;
+}
+
+// Pass 1: place every sample (FP64, AddDataSample arithmetic) and histogram
+// the (layer, uv tile) bins of all tiles its footprint touches. Pass 2
+// (SCATTER) repeats the placement and writes the records to their bins.
+// Samples are visited in row groups: group j holds positions
+// [j * 1024, (j + 1) * 1024) of the samples in the order of `rowOrder` (rows
+// sorted by their middle channel's bin, channels in order within a row), so
+// the rows of a group share their bins. The lanes of a run of consecutive
+// samples in one bin count it once; the runs of a group add their lengths in
+// an LDS table, and the table adds one count per distinct bin to memory
+// (the scatter: takes one block of positions per bin and hands each run its
+// part). Workgroups visit groups in the order j = q * step mod ngroups
+// (step coprime to ngroups), so groups in flight at once lie far apart in the
+// sorted order and their bins rarely coincide. The gridded / off-grid /
+// off-layer counters are summed per workgroup and added to memory once.
+template <bool SCATTER, bool KEYS = false>
+__global__ void __launch_bounds__(kThreads)
+binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
+          const uint32_t* __restrict__ rowOrder, size_t nrow, int nchan, uint32_t ngroups,
+          uint32_t step, GeomArgs ga, KindArgs k) {
+  __shared__ uint32_t slotBin[kBinSlots];
+  __shared__ uint32_t slotCount[kBinSlots];  // scatter: then the bin's first position
   __shared__ unsigned long long sCount[3];
+  for (int t = threadIdx.x; t < kBinSlots; t += blockDim.x) {
+    slotBin[t] = kNoBin;
+    slotCount[t] = 0u;
+  }
   if (!SCATTER && threadIdx.x < 3) sCount[threadIdx.x] = 0ull;
   __syncthreads();
-  const size_t n = nrow * size_t(nchan);
+  const uint32_t n = uint32_t(nrow * size_t(nchan));
   const int lane = threadIdx.x & 63;
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t base = size_t(blockIdx.x) * blockDim.x; base < n; base += stride) {
-    const size_t i = base + threadIdx.x;
-    Placement p;
-    p.layer = -1;
-    p.onGrid = 0;
-    if (i < n) p = placeSample(ga, k.lm, uvw, wl, i, nchan);
-    const bool ok = i < n && p.layer >= 0 && p.onGrid;
-    if (!SCATTER) {
-      waveCount(&sCount[0], ok);
-      waveCount(&sCount[1], i < n && p.layer >= 0 && !p.onGrid);
-      waveCount(&sCount[2], i < n && p.layer < 0);
-    }
-    TileSpan sx, sy;
-    sx.tA = sx.tB = sy.tA = sy.tB = -1;
-    sx.xoA = sx.xoB = sy.xoA = sy.xoB = 0;
-    if (ok) footprintSpans(ga, p, sx, sy);
-    // the four emissions first, then their runs, then all atomics at once
-    // (the scatter's returning atomics are in flight together)
-    uint32_t bin[4];
-    uint2 rec[4];
-    bool has[4];
-    LaneRun run[4];
+  for (uint32_t q = blockIdx.x; q < ngroups; q += gridDim.x) {
+    const uint32_t grp = uint32_t((uint64_t(q) * step) % ngroups);
+    // per sample and emission: the record, and on the run head its table
+    // slot and offset (or, past a full table, its position in memory)
+    uint2 rec[kGroupPer][4];
+    uint32_t meta[kGroupPer][4];  // bit 0 has, bit 1 direct, bits 2-7 head lane, 8-17 slot
+    uint32_t loc[kGroupPer][4];   // head: offset within the slot, or the position
+    uint32_t binKey[KEYS ? kGroupPer : 1][4];  // KEYS: the bin (for the sort keys)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      bin[e] = 0xFFFFFFFFu;
-      rec[e] = make_uint2(0u, 0u);
-      has[e] = ok && emission(ga, p, sx, sy, e, uint32_t(i), bin[e], rec[e]);
-      run[e] = laneRun(bin[e], has[e]);
-    }
-    if (!SCATTER) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (has[e] && run[e].head == lane) atomicAdd(&k.counts[bin[e]], uint32_t(run[e].len));
-    } else {
-      uint32_t pos[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        pos[e] = 0;
-        if (has[e] && run[e].head == lane)
-          pos[e] = atomicAdd(&k.cursor[bin[e]], uint32_t(run[e].len));
+    for (int qq = 0; qq < kGroupPer; ++qq) {
+      const uint32_t s = grp * uint32_t(kGroupSamples) + uint32_t(qq * kThreads) + threadIdx.x;
+      const bool inRange = s < n;
+      uint32_t idx = 0u;
+      Placement p;
+      p.layer = -1;
+      p.onGrid = 0;
+      if (inRange) {
+        const uint32_t sr = s / uint32_t(nchan);
+        const uint32_t ch = s - sr * uint32_t(nchan);
+        const uint32_t row = rowOrder[sr];
+        idx = row * uint32_t(nchan) + ch;
+        const double lambda = wl[ch];
+        p = place(ga, k.lm, uvw[size_t(row) * 3] / lambda, uvw[size_t(row) * 3 + 1] / lambda,
+                  uvw[size_t(row) * 3 + 2] / lambda);
       }
+      const bool ok = inRange && p.layer >= 0 && p.onGrid;
+      if (!SCATTER) {
+        waveCount(&sCount[0], ok);
+        waveCount(&sCount[1], inRange && p.layer >= 0 && !p.onGrid);
+        waveCount(&sCount[2], inRange && p.layer < 0);
+      }
+      TileSpan sx, sy;
+      sx.tA = sx.tB = sy.tA = sy.tB = -1;
+      sx.xoA = sx.xoB = sy.xoA = sy.xoB = 0;
+      if (ok) footprintSpans(ga, p, sx, sy);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const uint32_t at = __shfl(pos[e], run[e].head) + uint32_t(lane - run[e].head);
-        if (has[e]) {
-          k.recSorted[at] = rec[e];
-          if (k.sortKeys) k.sortKeys[at] = (bin[e] << 12) | (rec[e].y & 4095u);
+        uint32_t bin = kNoBin;
+        uint2 r = make_uint2(0u, 0u);
+        const bool has = ok && emission(ga, p, sx, sy, e, idx, bin, r);
+        const LaneRun run = laneRun(bin, has);
+        uint32_t m = has ? 1u : 0u;
+        uint32_t l = 0u;
+        if (has && run.head == lane) {
+          const int slot = binSlot(slotBin, bin);
+          if (slot >= 0) {
+            l = atomicAdd(&slotCount[slot], uint32_t(run.len));
+            m |= uint32_t(slot) << 8;
+          } else {
+            m |= 2u;
+            if (!SCATTER)
+              atomicAdd(&k.counts[bin], uint32_t(run.len));
+            else
+              l = atomicAdd(&k.cursor[bin], uint32_t(run.len));
+          }
         }
+        m |= uint32_t(run.head) << 2;
+        rec[qq][e] = r;
+        meta[qq][e] = m;
+        loc[qq][e] = l;
+        if constexpr (KEYS) binKey[qq][e] = bin;
       }
+    }
+    __syncthreads();
+    // one count per distinct bin of the group
+    for (int t = threadIdx.x; t < kBinSlots; t += blockDim.x) {
+      const uint32_t bin = slotBin[t];
+      if (bin == kNoBin) continue;
+      if (!SCATTER) {
+        atomicAdd(&k.counts[bin], slotCount[t]);
+        slotBin[t] = kNoBin;
+        slotCount[t] = 0u;
+      } else {
+        slotCount[t] = atomicAdd(&k.cursor[bin], slotCount[t]);
+      }
+    }
+    __syncthreads();
+    if (SCATTER) {
+#pragma unroll
+      for (int qq = 0; qq < kGroupPer; ++qq)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t m = meta[qq][e];
+          const int head = int((m >> 2) & 63u);
+          uint32_t pos = 0u;
+          if ((m & 1u) && head == lane)
+            pos = (m & 2u) ? loc[qq][e] : slotCount[(m >> 8) & 1023u] + loc[qq][e];
+          const uint32_t at = __shfl(pos, head) + uint32_t(lane - head);
+          // (a position past the records is never written; the plan check
+          // reports the bins it belongs to)
+          if ((m & 1u) && at < k.nRecords) {
+            k.recSorted[at] = rec[qq][e];
+            if constexpr (KEYS) k.sortKeys[at] = (binKey[qq][e] << 12) | (rec[qq][e].y & 4095u);
+          }
+        }
+      __syncthreads();
+      for (int t = threadIdx.x; t < kBinSlots; t += blockDim.x) {
+        slotBin[t] = kNoBin;
+        slotCount[t] = 0u;
+      }
+      __syncthreads();
     }
   }
   if (!SCATTER) {
@@ -344,6 +455,17 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t 
     if (threadIdx.x < 3 && sCount[threadIdx.x])
       atomicAdd(&k.counters[threadIdx.x], sCount[threadIdx.x]);
   }
+}
+
+// Plan check: every bin's scatter cursor ends at the next bin's first record
+// (each record was placed exactly once); counts the bins where it does not.
+__global__ void __launch_bounds__(kThreads)
+planCheckKernel(const uint32_t* __restrict__ cursor, const uint32_t* __restrict__ binStart,
+                size_t nbins, unsigned long long* __restrict__ bad) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool wrong = b < nbins && cursor[b] != binStart[b + 1];
+  const int cnt = __syncthreads_count(wrong);
+  if (threadIdx.x == 0 && cnt) atomicAdd(bad, (unsigned long long)cnt);
 }
 
 // Work items per bin: a bin of c > 0 records is processed as ceil(c / chunk)
@@ -2072,8 +2194,41 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
 
   const GeomArgs ga = geomArgs(g);
   pl.chunk = workChunk(g, n);
+  // rows in the order of their middle channel's bin (the layer map of the
+  // real images if there are any)
+  int keyKind = 1;
+  for (int i = 0; i < g->nImg; ++i)
+    if (!g->isComplex[i]) keyKind = 0;
+  HWS_TRY(ensureAlloc(g, pl.rowKeys, pl.rowKeysCapacity, std::max<size_t>(nrow, 1), "row keys"));
+  HWS_TRY(ensureAlloc(g, pl.rowKeysAlt, pl.rowKeysAltCapacity, std::max<size_t>(nrow, 1),
+                      "row keys"));
+  HWS_TRY(ensureAlloc(g, pl.rowOrder, pl.rowOrderCapacity, std::max<size_t>(nrow, 1),
+                      "row order"));
+  HWS_TRY(ensureAlloc(g, pl.rowOrderAlt, pl.rowOrderAltCapacity, std::max<size_t>(nrow, 1),
+                      "row order"));
+  if (n > 0) {
+    hipLaunchKernelGGL(rowKeyKernel, dim3(blocksFor(nrow)), dim3(kThreads), 0, g->stream, dUvw,
+                       dWl, nrow, int(nchan), ga, layerMap(g, keyKind), pl.rowKeys, pl.rowOrder);
+    HWS_TRY(checkHip(g, hipGetLastError(), "row key launch"));
+    HWS_TRY(sortRows(g, pl.rowKeys, pl.rowKeysAlt, pl.rowOrder, pl.rowOrderAlt, nrow));
+  }
+  // row groups of 1024 samples, visited with a stride coprime to their number
+  const uint32_t ngroups = uint32_t((n + kGroupSamples - 1) / kGroupSamples);
+  uint32_t step = 1;
+  if (ngroups > 2) {
+    step = uint32_t(double(ngroups) * 0.6180339887498949);
+    auto gcd = [](uint32_t a, uint32_t b) {
+      while (b) {
+        const uint32_t t = a % b;
+        a = b;
+        b = t;
+      }
+      return a;
+    };
+    while (gcd(step, ngroups) != 1u) ++step;
+  }
   const unsigned binBlocks =
-      unsigned(std::max<size_t>(1, std::min<size_t>(blocksFor(n), size_t(persistentBlocks(g, 16)))));
+      unsigned(std::max<size_t>(1, std::min<size_t>(ngroups, size_t(persistentBlocks(g, 8)))));
   for (int kind = 0; kind < 2; ++kind) {
     KindPlan& kp = pl.kinds[kind];
     kp.kind = kind;
@@ -2088,10 +2243,10 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     HWS_TRY(ensureAlloc(g, kp.itemStart, kp.itemCapacity, pl.nbins + 1, "plan item offsets"));
     HWS_TRY(launchFill(g, kp.counts, 0u, pl.nbins + 1));
     KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind,
-                nullptr};
+                nullptr, 0u};
     if (n > 0) {
       hipLaunchKernelGGL(binKernel<false>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
-                         dUvw, dWl, nrow, int(nchan), ga, ka);
+                         dUvw, dWl, pl.rowOrder, nrow, int(nchan), ngroups, step, ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin count launch"));
     }
     HWS_TRY(exclusiveScan(g, kp.counts, kp.binStart, pl.nbins, pl.scanScratch));
@@ -2153,6 +2308,7 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                                        hipMemcpyDeviceToDevice, g->stream), "cursor copy"));
     ka.cursor = kp.counts;
     ka.recSorted = kp.recSorted;
+    ka.nRecords = total;
     // sort keys bin << 12 | origin need 12 + log2(nbins) <= 32 bits
     int binBits = 0;
     while ((size_t(1) << binBits) < pl.nbins) ++binBits;
@@ -2166,10 +2322,19 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
       ka.sortKeys = kp.keys;
     }
     if (n > 0) {
-      hipLaunchKernelGGL(binKernel<true>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
-                         dUvw, dWl, nrow, int(nchan), ga, ka);
+      if (sortByOrigin)
+        hipLaunchKernelGGL((binKernel<true, true>), dim3(binBlocks), dim3(kThreads), 0,
+                           g->stream, dUvw, dWl, pl.rowOrder, nrow, int(nchan), ngroups, step,
+                           ga, ka);
+      else
+        hipLaunchKernelGGL((binKernel<true, false>), dim3(binBlocks), dim3(kThreads), 0,
+                           g->stream, dUvw, dWl, pl.rowOrder, nrow, int(nchan), ngroups, step,
+                           ga, ka);
       HWS_TRY(checkHip(g, hipGetLastError(), "bin scatter launch"));
     }
+    hipLaunchKernelGGL(planCheckKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
+                       kp.counts, kp.binStart, pl.nbins, pl.counters + 12 + kind);
+    HWS_TRY(checkHip(g, hipGetLastError(), "plan check launch"));
     if (sortByOrigin && n > 0) {
       uint32_t* k0 = kp.keys;
       uint2* r0 = kp.recSorted;
@@ -2179,6 +2344,12 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
       if (kp.recSorted != r0) std::swap(kp.capacity, kp.altCapacity);
     }
   }
+  unsigned long long bad[2] = {0ull, 0ull};
+  HWS_TRY(checkHip(g, hipMemcpyAsync(bad, pl.counters + 12, sizeof(bad), hipMemcpyDeviceToHost,
+                                     g->stream), "plan check"));
+  HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "plan check"));
+  if (bad[0] || bad[1])
+    return setLastError(g, HWS_ERR_BAD_STATE, "binning plan inconsistent: records misplaced");
   pl.valid = true;
   return HWS_OK;
 }
