@@ -295,8 +295,6 @@ rowKeyKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size
   rows[r] = uint32_t(r);
 }
 
-constexpr int kGroupSamples = 1024;  // samples of one row group (4 per thread)
-constexpr int kGroupPer = kGroupSamples / kThreads;
 constexpr int kBinSlots = 1024;      // LDS table of one group's bins
 constexpr uint32_t kNoBin = 0xFFFFFFFFu;
 
@@ -331,7 +329,7 @@ __device__ __forceinline__ int binSlot(uint32_t* keys, uint32_t bin) {
 // (step coprime to ngroups), so groups in flight at once lie far apart in the
 // sorted order and their bins rarely coincide. The gridded / off-grid /
 // off-layer counters are summed per workgroup and added to memory once.
-template <bool SCATTER, bool KEYS = false>
+template <bool SCATTER, bool KEYS, int PER>
 __global__ void __launch_bounds__(kThreads)
 binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
           const uint32_t* __restrict__ rowOrder, size_t nrow, int nchan, uint32_t ngroups,
@@ -351,13 +349,13 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
     const uint32_t grp = uint32_t((uint64_t(q) * step) % ngroups);
     // per sample and emission: the record, and on the run head its table
     // slot and offset (or, past a full table, its position in memory)
-    uint2 rec[kGroupPer][4];
-    uint32_t meta[kGroupPer][4];  // bit 0 has, bit 1 direct, bits 2-7 head lane, 8-17 slot
-    uint32_t loc[kGroupPer][4];   // head: offset within the slot, or the position
-    uint32_t binKey[KEYS ? kGroupPer : 1][4];  // KEYS: the bin (for the sort keys)
+    uint2 rec[PER][4];
+    uint32_t meta[PER][4];  // bit 0 has, bit 1 direct, bits 2-7 head lane, 8-17 slot
+    uint32_t loc[PER][4];   // head: offset within the slot, or the position
+    uint32_t binKey[KEYS ? PER : 1][4];  // KEYS: the bin (for the sort keys)
 #pragma unroll
-    for (int qq = 0; qq < kGroupPer; ++qq) {
-      const uint32_t s = grp * uint32_t(kGroupSamples) + uint32_t(qq * kThreads) + threadIdx.x;
+    for (int qq = 0; qq < PER; ++qq) {
+      const uint32_t s = grp * uint32_t(PER * kThreads) + uint32_t(qq * kThreads) + threadIdx.x;
       const bool inRange = s < n;
       uint32_t idx = 0u;
       Placement p;
@@ -426,7 +424,7 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
     __syncthreads();
     if (SCATTER) {
 #pragma unroll
-      for (int qq = 0; qq < kGroupPer; ++qq)
+      for (int qq = 0; qq < PER; ++qq)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const uint32_t m = meta[qq][e];
@@ -2212,23 +2210,34 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     HWS_TRY(checkHip(g, hipGetLastError(), "row key launch"));
     HWS_TRY(sortRows(g, pl.rowKeys, pl.rowKeysAlt, pl.rowOrder, pl.rowOrderAlt, nrow));
   }
-  // row groups of 1024 samples, visited with a stride coprime to their number
-  const uint32_t ngroups = uint32_t((n + kGroupSamples - 1) / kGroupSamples);
-  uint32_t step = 1;
-  if (ngroups > 2) {
-    step = uint32_t(double(ngroups) * 0.6180339887498949);
-    auto gcd = [](uint32_t a, uint32_t b) {
-      while (b) {
-        const uint32_t t = a % b;
-        a = b;
-        b = t;
-      }
-      return a;
-    };
-    while (gcd(step, ngroups) != 1u) ++step;
-  }
-  const unsigned binBlocks =
-      unsigned(std::max<size_t>(1, std::min<size_t>(ngroups, size_t(persistentBlocks(g, 8)))));
+  // row groups of PER * 256 samples, visited with a stride coprime to their
+  // number: the count pass takes 16 samples per thread and group, the
+  // scatter (which keeps its records in registers until the group's
+  // positions are known) 2 (measured best of 2-16 and 1-4 at the 4096^2
+  // benchmark)
+  constexpr int countPer = 16, scatterPer = 2;
+  auto groupsOf = [&](int per, uint32_t& ngroups, uint32_t& step, unsigned& blocks) {
+    const size_t gs = size_t(per) * kThreads;
+    ngroups = uint32_t((n + gs - 1) / gs);
+    step = 1;
+    if (ngroups > 2) {
+      step = uint32_t(double(ngroups) * 0.6180339887498949);
+      auto gcd = [](uint32_t a, uint32_t b) {
+        while (b) {
+          const uint32_t t = a % b;
+          a = b;
+          b = t;
+        }
+        return a;
+      };
+      while (gcd(step, ngroups) != 1u) ++step;
+    }
+    blocks = unsigned(std::max<size_t>(1, std::min<size_t>(ngroups, size_t(persistentBlocks(g, 8)))));
+  };
+  uint32_t cGroups, cStep, sGroups, sStep;
+  unsigned cBlocks, sBlocks;
+  groupsOf(countPer, cGroups, cStep, cBlocks);
+  groupsOf(scatterPer, sGroups, sStep, sBlocks);
   for (int kind = 0; kind < 2; ++kind) {
     KindPlan& kp = pl.kinds[kind];
     kp.kind = kind;
@@ -2245,8 +2254,11 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind,
                 nullptr, 0u};
     if (n > 0) {
-      hipLaunchKernelGGL(binKernel<false>, dim3(binBlocks), dim3(kThreads), 0, g->stream,
-                         dUvw, dWl, pl.rowOrder, nrow, int(nchan), ngroups, step, ga, ka);
+#define HWS_COUNT(P)                                                                      \
+  hipLaunchKernelGGL((binKernel<false, false, P>), dim3(cBlocks), dim3(kThreads), 0, g->stream, \
+                     dUvw, dWl, pl.rowOrder, nrow, int(nchan), cGroups, cStep, ga, ka)
+      HWS_COUNT(countPer);
+#undef HWS_COUNT
       HWS_TRY(checkHip(g, hipGetLastError(), "bin count launch"));
     }
     HWS_TRY(exclusiveScan(g, kp.counts, kp.binStart, pl.nbins, pl.scanScratch));
@@ -2322,14 +2334,14 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
       ka.sortKeys = kp.keys;
     }
     if (n > 0) {
+#define HWS_SCATTER(K, P)                                                                   \
+  hipLaunchKernelGGL((binKernel<true, K, P>), dim3(sBlocks), dim3(kThreads), 0, g->stream,       \
+                     dUvw, dWl, pl.rowOrder, nrow, int(nchan), sGroups, sStep, ga, ka)
       if (sortByOrigin)
-        hipLaunchKernelGGL((binKernel<true, true>), dim3(binBlocks), dim3(kThreads), 0,
-                           g->stream, dUvw, dWl, pl.rowOrder, nrow, int(nchan), ngroups, step,
-                           ga, ka);
+        HWS_SCATTER(true, scatterPer);
       else
-        hipLaunchKernelGGL((binKernel<true, false>), dim3(binBlocks), dim3(kThreads), 0,
-                           g->stream, dUvw, dWl, pl.rowOrder, nrow, int(nchan), ngroups, step,
-                           ga, ka);
+        HWS_SCATTER(false, scatterPer);
+#undef HWS_SCATTER
       HWS_TRY(checkHip(g, hipGetLastError(), "bin scatter launch"));
     }
     hipLaunchKernelGGL(planCheckKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
