@@ -314,7 +314,7 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
   const double phi = w * np;
   const float f = float(phi - rint(phi));
   float sn, cs;
-  sincospif(2.0f * f, &sn, &cs);
+  turnSinCos(f, sn, cs);
   if (ms.cplx) return make_float2(re * cs + imv * sn, imv * cs - re * sn);
   return make_float2(re * cs, -re * sn);
 }

@@ -81,7 +81,7 @@ struct GeomArgs {
   int tileSize, tilesX, tilesY;
   int tileShift;     // log2(tileSize) when it is a power of two, else -1
   int rotate;        // apply the l/m shift phase rotation to samples
-  int fastPhase;     // rotation phasor from an FP64-reduced FP32 sincospi
+  int fastPhase;     // rotation phasor from an FP64-reduced FP32 turnSinCos
   double lShift, mShift;
   int blocked;       // layer storage layout (cellIndex)
 };
@@ -700,7 +700,7 @@ struct GridArgs {
 // (gridding/msgridderbase.cpp:692-704) with shiftFactor = sign * 2 pi
 // (u l_shift + v m_shift) and rad = shiftFactor / lambda. The argument is
 // FP64; the mixed precision reduces it to a fraction of a turn in FP64 and
-// takes an FP32 sincospi, the reference precision an FP64 sincos.
+// takes the FP32 turnSinCos, the reference precision an FP64 sincos.
 __device__ __forceinline__ float2 rotationPhasor(const GridArgs& a,
                                                  const double* __restrict__ uvw,
                                                  const double* __restrict__ wl, uint32_t idx,
@@ -711,7 +711,7 @@ __device__ __forceinline__ float2 rotationPhasor(const GridArgs& a,
   if (a.ga.fastPhase) {
     const float f = float(turns - rint(turns));
     float sf, cf;
-    sincospif(2.0f * f, &sf, &cf);
+    turnSinCos(f, sf, cf);
     return make_float2(cf, sf);
   }
   double s, c;
@@ -1816,7 +1816,7 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
       const double phi = w * np;
       const float f = float(phi - rint(phi));
       float sf, cf;
-      sincospif(2.0f * f, &sf, &cf);
+      turnSinCos(f, sf, cf);
       // exp(-2 pi i phi): sin = -sf, cos = cf
       re = F.x * cf + F.y * sf;
       im = -F.x * sf + F.y * cf;
@@ -1966,7 +1966,7 @@ imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
       const double phi = w * np;
       const float f = float(phi - rint(phi));
       float sn, cs;
-      sincospif(2.0f * f, &sn, &cs);
+      turnSinCos(f, sn, cs);
       if (cplx)
         out = make_float2(re * cs + imv * sn, imv * cs - re * sn);
       else
