@@ -395,25 +395,32 @@ def main():
     value = w.nvis / ms_step
 
     # dominant kernel class: the per-layer 2-D FFT of the forward (predict)
-    # direction (fftRowsBlockedKernel + fftColsBlockedSparseKernel). Its
-    # algorithmic bytes per layer transform: the row pass reads and writes
-    # every element (16 B), the column pass reads every element (8 B) and
-    # writes the uv tiles the degridding reads (8 B per cell of a needed
-    # tile), counted by the plan
+    # direction (fftRowsFromModelKernel + fftColsBlockedSparseKernel). Its
+    # algorithmic bytes per layer transform: the row pass computes its input
+    # from the model image (read once per image: 4 B per pixel for a real
+    # image, 8 B for a complex one) and writes every element (8 B), the
+    # column pass reads every element (8 B) and writes the uv tiles the
+    # degridding reads (8 B per cell of a needed tile), counted by the plan
     n_fft_2d = 4 * Workload.NW  # layer transforms per direction (4 images x 64 layers)
     fft_ms = ph["fft_pred"]
     HW = Workload.W * Workload.H
     need = st_pred["tiles_needed"] if st_pred else [0, 0]
     occ = st_pred["tiles_occupied"] if st_pred else [0, 0]
     ntiles = st_pred["tiles_total"] if st_pred else 0
+    rlive = st_pred["tile_rows_live"] if st_pred else [0, 0]
+    nrows_t = st_pred["tile_rows_total"] if st_pred else 0
     # images per kind in this workload: XX, YY real; XY, YX complex
     fwd_bytes = 0.0
     inv_bytes = 0.0
     for k, nimg in ((0, 2), (1, 2)):
         frac_need = need[k] / ntiles if ntiles else 1.0
         frac_occ = occ[k] / ntiles if ntiles else 1.0
-        fwd_bytes += nimg * Workload.NW * HW * (24 + 8 * frac_need)
-        inv_bytes += nimg * Workload.NW * HW * (8 * frac_occ + 24)
+        fwd_bytes += nimg * Workload.NW * HW * (16 + 8 * frac_need) + nimg * HW * 4 * (1 + k)
+        # backward: the row pass reads the tiles holding records and writes
+        # every element, the column pass reads the tile rows holding records
+        # and writes every element
+        frac_rows = (rlive[k] / nrows_t) if nrows_t else 1.0
+        inv_bytes += nimg * Workload.NW * HW * (16 + 8 * frac_occ + 8 * frac_rows)
     bytes_per_fft = fwd_bytes / n_fft_2d
     fft_gbs = fwd_bytes / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else 0.0
     inv_gbs = inv_bytes / (ph["fft_inv"] * 1e-3) / 1e9 if ph["fft_inv"] > 0 else 0.0
@@ -429,7 +436,7 @@ def main():
     traffic = fft_traffic_from_profile()
     roofline = {
         "bound": "hbm",
-        "kernel": "forward layer 2-D FFT, blocked layout (fftRowsBlockedKernel + "
+        "kernel": "forward layer 2-D FFT, blocked layout (fftRowsFromModelKernel + "
                   "fftColsBlockedSparseKernel)",
         "achieved": round(fft_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(fft_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -125,6 +125,10 @@ typedef struct hws_stats {
   uint64_t tiles_total;
   uint64_t tiles_occupied[2];
   uint64_t tiles_needed[2];
+  /* (layer, tile row) pairs over all layers of the plan, and those with a
+   * tile holding records: the rows the backward column pass reads. */
+  uint64_t tile_rows_total;
+  uint64_t tile_rows_live[2];
 } hws_stats;
 
 /* Replaces: WStackingGridder<T>::WStackingGridder (gridding/wstackinggridder.h:120-122,

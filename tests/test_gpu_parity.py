@@ -408,6 +408,18 @@ def test_split_bins_match_oracle(hws, oracle, precision):
     d.invert(d_vis, 1.0, out)
     st = d.stats()
     assert st["chunks"] > st["bins_used"] + 50  # large bins were split into extra items
+    # tile counters against the oracle's gridded layers: a (layer, uv tile)
+    # bin holds records exactly when the oracle grids a non-zero cell into
+    # that tile (random visibilities, kernel taps all non-zero)
+    tiles_per_layer = st["tiles_total"] // nw
+    t_edge = w // int(round(np.sqrt(tiles_per_layer)))
+    nz = np.stack([np.abs(np.asarray(lo)) > 0 for lo in lay_o])  # [layer, y, x]
+    nt = w // t_edge
+    tiles = nz.reshape(nw, nt, t_edge, nt, t_edge).any(axis=(2, 4))  # [layer, ty, tx]
+    assert st["tiles_occupied"] == [int(tiles.sum()), 0]
+    assert st["tile_rows_total"] == nw * nt
+    assert st["tile_rows_live"] == [int(tiles.any(axis=2).sum()), 0]
+    assert st["tiles_occupied"][0] <= st["tiles_needed"][0] <= st["tiles_total"]
     assert rel(out.cpu().numpy(), re) < 1e-6
     rng = np.random.default_rng(3)
     model = rng.standard_normal((2, w, w)).astype(np.float32)

@@ -45,7 +45,8 @@ class HwsStats(C.Structure):
         ("ms_bin", C.c_double), ("ms_grid", C.c_double), ("ms_fft", C.c_double),
         ("ms_project", C.c_double), ("ms_total", C.c_double),
         ("tiles_total", C.c_uint64), ("tiles_occupied", C.c_uint64 * 2),
-        ("tiles_needed", C.c_uint64 * 2),
+        ("tiles_needed", C.c_uint64 * 2), ("tile_rows_total", C.c_uint64),
+        ("tile_rows_live", C.c_uint64 * 2),
     ]
 
 

@@ -319,6 +319,15 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
   return make_float2(re * cs, -re * sn);
 }
 
+// A store the pass does not read back: marked non-temporal, so the stream of
+// layer writes does not evict what the caches hold (the model image the
+// forward rows pass reads for every layer).
+__device__ __forceinline__ void streamStore(float2* p, float2 v) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 x = {v.x, v.y};
+  __builtin_nontemporal_store(x, reinterpret_cast<f2*>(p));
+}
+
 // MODEL (rows pass of a forward transform): the first stage computes its
 // inputs from the model image (ms, w; y0 = the row block's first row)
 // instead of reading the layer.
@@ -370,7 +379,7 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
       if (LAST) {
         // columns pass of a forward transform: tiles nobody reads are not written
         if (live && !liveIn && !live[pos >> tshift]) continue;
-        g[blockedAddr<ROWS>(pos, f, W)] = v[q][r];
+        streamStore(&g[blockedAddr<ROWS>(pos, f, W)], v[q][r]);
       }
       else
         lds[padIdx(pos * 4 + f)] = v[q][r];

@@ -2306,6 +2306,8 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                        kp.occ, occWords, pl.counters + 8 + kind);
     hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(occWords)), dim3(kThreads), 0, g->stream,
                        kp.need, occWords, pl.counters + 10 + kind);
+    hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(rowWords)), dim3(kThreads), 0, g->stream,
+                       kp.rowLive, rowWords, pl.counters + 14 + kind);
     HWS_TRY(checkHip(g, hipGetLastError(), "tile count launch"));
     hipLaunchKernelGGL(itemBinsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
                        kp.itemStart, pl.nbins, kp.itemBin);
