@@ -399,8 +399,9 @@ def main():
     # algorithmic bytes per layer transform: the row pass computes its input
     # from the model image (read once per image: 4 B per pixel for a real
     # image, 8 B for a complex one) and writes every element (8 B), the
-    # column pass reads every element (8 B) and writes the uv tiles the
-    # degridding reads (8 B per cell of a needed tile), counted by the plan
+    # column pass reads the tile columns holding a needed tile (8 B per
+    # element) and writes the uv tiles the degridding reads (8 B per cell of
+    # a needed tile); the fractions are counted by the plan
     n_fft_2d = 4 * Workload.NW  # layer transforms per direction (4 images x 64 layers)
     fft_ms = ph["fft_pred"]
     HW = Workload.W * Workload.H
@@ -409,13 +410,17 @@ def main():
     ntiles = st_pred["tiles_total"] if st_pred else 0
     rlive = st_pred["tile_rows_live"] if st_pred else [0, 0]
     nrows_t = st_pred["tile_rows_total"] if st_pred else 0
+    cneed = st_pred["tile_cols_needed"] if st_pred else [0, 0]
+    ncols_t = st_pred["tile_cols_total"] if st_pred else 0
     # images per kind in this workload: XX, YY real; XY, YX complex
     fwd_bytes = 0.0
     inv_bytes = 0.0
     for k, nimg in ((0, 2), (1, 2)):
         frac_need = need[k] / ntiles if ntiles else 1.0
         frac_occ = occ[k] / ntiles if ntiles else 1.0
-        fwd_bytes += nimg * Workload.NW * HW * (16 + 8 * frac_need) + nimg * HW * 4 * (1 + k)
+        frac_cols = (cneed[k] / ncols_t) if ncols_t else 1.0
+        fwd_bytes += (nimg * Workload.NW * HW * (8 + 8 * frac_cols + 8 * frac_need) +
+                      nimg * HW * 4 * (1 + k))
         # backward: the row pass reads the tiles holding records and writes
         # every element, the column pass reads the tile rows holding records
         # and writes every element
@@ -446,6 +451,9 @@ def main():
         "tiles_needed_fraction": [round(n / ntiles, 4) if ntiles else None for n in need],
         "backward_transform_gbs": round(inv_gbs, 1),
         "tiles_occupied_fraction": [round(n / ntiles, 4) if ntiles else None for n in occ],
+        "tile_columns_needed_fraction": [round(n / ncols_t, 4) if ncols_t else None
+                                         for n in cneed],
+        "tile_rows_live_fraction": [round(n / nrows_t, 4) if nrows_t else None for n in rlive],
     }
 
     plain = None

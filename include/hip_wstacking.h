@@ -129,6 +129,10 @@ typedef struct hws_stats {
    * tile holding records: the rows the backward column pass reads. */
   uint64_t tile_rows_total;
   uint64_t tile_rows_live[2];
+  /* (layer, tile column) pairs, and those with a tile a degridding subgrid
+   * reads: the columns the forward column pass transforms. */
+  uint64_t tile_cols_total;
+  uint64_t tile_cols_needed[2];
 } hws_stats;
 
 /* Replaces: WStackingGridder<T>::WStackingGridder (gridding/wstackinggridder.h:120-122,

@@ -419,6 +419,14 @@ def test_split_bins_match_oracle(hws, oracle, precision):
     assert st["tiles_occupied"] == [int(tiles.sum()), 0]
     assert st["tile_rows_total"] == nw * nt
     assert st["tile_rows_live"] == [int(tiles.any(axis=2).sum()), 0]
+    # needed tiles: a tile is read by the subgrids of its own bin and of the
+    # bins to its left, above and above-left (the subgrid reaches K - 1 cells
+    # into the next tiles, wrapping round the grid)
+    need = (tiles | np.roll(tiles, 1, axis=2) | np.roll(tiles, 1, axis=1) |
+            np.roll(np.roll(tiles, 1, axis=1), 1, axis=2))
+    assert st["tiles_needed"] == [int(need.sum()), 0]
+    assert st["tile_cols_total"] == nw * nt
+    assert st["tile_cols_needed"] == [int(need.any(axis=1).sum()), 0]
     assert st["tiles_occupied"][0] <= st["tiles_needed"][0] <= st["tiles_total"]
     assert rel(out.cpu().numpy(), re) < 1e-6
     rng = np.random.default_rng(3)

@@ -46,7 +46,8 @@ class HwsStats(C.Structure):
         ("ms_project", C.c_double), ("ms_total", C.c_double),
         ("tiles_total", C.c_uint64), ("tiles_occupied", C.c_uint64 * 2),
         ("tiles_needed", C.c_uint64 * 2), ("tile_rows_total", C.c_uint64),
-        ("tile_rows_live", C.c_uint64 * 2),
+        ("tile_rows_live", C.c_uint64 * 2), ("tile_cols_total", C.c_uint64),
+        ("tile_cols_needed", C.c_uint64 * 2),
     ]
 
 

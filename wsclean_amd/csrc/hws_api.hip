@@ -925,8 +925,10 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
       s.tiles_occupied[k] = c[8 + k];
       s.tiles_needed[k] = c[10 + k];
       s.tile_rows_live[k] = c[14 + k];
+      s.tile_cols_needed[k] = c[16 + k];
     }
     s.tile_rows_total = g->plan.tilesX ? g->plan.nbins / size_t(g->plan.tilesX) : 0;
+    s.tile_cols_total = g->plan.tilesY ? g->plan.nbins / size_t(g->plan.tilesY) : 0;
     for (const auto& kp : g->plan.kinds)
       if (kp.active) s.chunks += kp.nWorkItems;
   }

@@ -16,7 +16,7 @@
 namespace hws {
 
 constexpr int kMaxImages = 4;
-constexpr int kPlanCounters = 16;
+constexpr int kPlanCounters = 18;
 constexpr int kMaxFftStages = 24;
 constexpr int kMaxKernelSize = 15;
 constexpr int kMaxOversampling = 1023;
@@ -172,7 +172,8 @@ struct Plan {
   // [0..2] gridded / off grid / off layer (real kind), [3..5] the same
   // (complex kind), [6 + kind] bins used, [8 + kind] occupied tiles,
   // [10 + kind] needed tiles, [12 + kind] bins failing the plan check,
-  // [14 + kind] live (layer, tile row) pairs
+  // [14 + kind] live (layer, tile row) pairs, [16 + kind] (layer, tile
+  // column) pairs holding a needed tile
   unsigned long long* counters = nullptr;
   uint32_t* queue = nullptr;  // work-queue head of the running grid launch
   // rows sorted by their middle channel's bin: sort keys and row indices

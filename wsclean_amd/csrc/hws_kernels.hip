@@ -549,6 +549,24 @@ popcountKernel(const uint32_t* __restrict__ bits, size_t words,
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, (unsigned long long)v);
 }
 
+// Counts the (layer, tile column) pairs with a set bit in some tile row
+// (thread per pair): the column blocks a sparse forward column pass reads.
+__global__ void __launch_bounds__(kThreads)
+liveColumnsKernel(const uint32_t* __restrict__ bits, size_t ncols, int tilesX, int tilesY,
+                  unsigned long long* __restrict__ out) {
+  const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool live = false;
+  if (c < ncols) {
+    const size_t layer = c / size_t(tilesX), tx = c - layer * size_t(tilesX);
+    for (int ty = 0; ty < tilesY && !live; ++ty) {
+      const size_t b = (layer * size_t(tilesY) + size_t(ty)) * size_t(tilesX) + tx;
+      live = (bits[b >> 5] >> (b & 31)) & 1u;
+    }
+  }
+  const unsigned long long m = __ballot(live);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+}
+
 // Need bits: tile (tx, ty) of a layer is read by the degridding subgrids of
 // bins (tx, ty), (tx - 1, ty), (tx, ty - 1) and (tx - 1, ty - 1) (the
 // (T + K - 1)^2 subgrid reaches K - 1 <= T cells into the next tiles,
@@ -2308,6 +2326,10 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                        kp.need, occWords, pl.counters + 10 + kind);
     hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(rowWords)), dim3(kThreads), 0, g->stream,
                        kp.rowLive, rowWords, pl.counters + 14 + kind);
+    const size_t ncolsBits = pl.nbins / size_t(pl.tilesY);
+    hipLaunchKernelGGL(liveColumnsKernel, dim3(blocksFor(ncolsBits)), dim3(kThreads), 0,
+                       g->stream, kp.need, ncolsBits, pl.tilesX, pl.tilesY,
+                       pl.counters + 16 + kind);
     HWS_TRY(checkHip(g, hipGetLastError(), "tile count launch"));
     hipLaunchKernelGGL(itemBinsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
                        kp.itemStart, pl.nbins, kp.itemBin);
