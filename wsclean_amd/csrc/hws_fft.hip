@@ -319,6 +319,10 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
   return make_float2(re * cs, -re * sn);
 }
 
+// Layers per workgroup of the model-fed rows pass: the workgroup reads the
+// model rows of its row block once from memory and then from the L2.
+constexpr int kModelLayers = 4;
+
 // A store the pass does not read back: marked non-temporal, so the stream of
 // layer writes does not evict what the caches hold (the model image the
 // forward rows pass reads for every layer).
@@ -420,13 +424,26 @@ fftRowsBlockedKernel(float2* __restrict__ data, int H, const float2* __restrict_
 // (H / 4, nbatch), batch entry b = layer ms.layer0 + b.
 template <int SIGN, int N, bool REF>
 __global__ void __launch_bounds__(N / 4)
-fftRowsFromModelKernel(float2* __restrict__ data, int H, const float2* __restrict__ tw,
-                       ModelSource ms) {
+fftRowsFromModelKernel(float2* __restrict__ data, int H, int nbatch,
+                       const float2* __restrict__ tw, ModelSource ms) {
   __shared__ float2 lds[4 * N + (4 * N) / 64];
-  float2* g = data + size_t(blockIdx.y) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
-  const double w = layerToW(ms.map, ms.layer0 + int(blockIdx.y));
-  blockedFft<SIGN, N, true, 1, REF ? 2 : 1>(lds, g, N, tw, threadIdx.x, nullptr, 0, true, &ms, w,
-                                            int(blockIdx.x) * 4, H);
+#pragma unroll 1
+  for (int li = 0; li < kModelLayers; ++li) {
+    const int bi = int(blockIdx.y) * kModelLayers + li;
+    if (bi >= nbatch) break;  // workgroup-uniform
+    // the empty asm gives the model pointers, pixel scale, twiddle table and
+    // thread index new values each iteration: nothing derived from them is
+    // held across the layers (the registers of one layer's transform are
+    // all the workgroup size allows)
+    ModelSource msl = ms;
+    const float2* twl = tw;
+    int tid = int(threadIdx.x);
+    asm volatile("" : "+s"(msl.re), "+s"(msl.im), "+s"(msl.dl), "+s"(twl), "+v"(tid));
+    float2* g = data + size_t(bi) * size_t(N) * size_t(H) + size_t(blockIdx.x) * 4 * N;
+    const double w = layerToW(ms.map, ms.layer0 + bi);
+    blockedFft<SIGN, N, true, 1, REF ? 2 : 1>(lds, g, N, twl, tid, nullptr, 0, true, &msl, w,
+                                              int(blockIdx.x) * 4, H);
+  }
 }
 
 // The same over layers whose uv tiles without samples were never written:
@@ -952,15 +969,16 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     } else if (sign < 0 && sparse) {
       if (model) {
         ok = false;
+        const dim3 mgrid(H / 4, (nbatch + kModelLayers - 1) / kModelLayers);
         switch (W) {
 #define HWS_MODEL_ROWS(NN)                                                                      \
   case NN:                                                                                      \
     if (model->precision == HWS_PRECISION_REFERENCE)                                            \
-      hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN, true>), dim3(H / 4, nbatch),           \
-                         dim3(NN / 4), 0, g->stream, data, H, twRow, *model);                   \
+      hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN, true>), mgrid, dim3(NN / 4), 0,         \
+                         g->stream, data, H, nbatch, twRow, *model);                            \
     else                                                                                        \
-      hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN, false>), dim3(H / 4, nbatch),          \
-                         dim3(NN / 4), 0, g->stream, data, H, twRow, *model);                   \
+      hipLaunchKernelGGL((fftRowsFromModelKernel<-1, NN, false>), mgrid, dim3(NN / 4), 0,        \
+                         g->stream, data, H, nbatch, twRow, *model);                            \
     ok = true;                                                                                  \
     break;
           HWS_MODEL_ROWS(256)
