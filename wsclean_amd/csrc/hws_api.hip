@@ -470,6 +470,7 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.need);
     freeDev(kp.rowLive);
     freeDev(kp.partials);
+    freeDev(kp.skipped);
     freeDev(kp.keys);
     freeDev(kp.keysAlt);
     freeDev(kp.recAlt);
@@ -834,7 +835,10 @@ hws_status hws_predict_planned_device(hws_gridder* g, const float* d_real,
   HWS_TRY(record(g, 2));
   HWS_TRY(ensureModel(g));
   HWS_TRY(initPrediction(g, d_real, d_imag));
-  HWS_TRY(launchFill(g, d_vis_out, 0x7FC00000u, pl.nrow * pl.nchan * size_t(g->nImg) * 2));
+  if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32)
+    HWS_TRY(launchFill(g, d_vis_out, 0x7FC00000u, pl.nrow * pl.nchan * size_t(g->nImg) * 2));
+  else
+    HWS_TRY(markSkipped(g, d_vis_out));  // the degridding writes every other value
   float ms[3] = {0, 0, 0};
   for (int pass = 0; pass < g->nPasses; ++pass) {
     // blocked binned form: the forward row pass computes the layers from the

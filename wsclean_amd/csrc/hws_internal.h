@@ -16,7 +16,7 @@
 namespace hws {
 
 constexpr int kMaxImages = 4;
-constexpr int kPlanCounters = 18;
+constexpr int kPlanCounters = 20;
 constexpr int kMaxFftStages = 24;
 constexpr int kMaxKernelSize = 15;
 constexpr int kMaxOversampling = 1023;
@@ -149,6 +149,11 @@ struct KindPlan {
   // bit per (layer, tile row): some tile of the row holds records
   uint32_t* rowLive = nullptr;
   size_t rowLiveCapacity = 0;
+  // samples of this kind that no record grids (off the grid or off every
+  // layer): the predict writes NaN for them [nSkipped]
+  uint32_t* skipped = nullptr;
+  uint64_t nSkipped = 0;
+  size_t skippedCapacity = 0;
   float2* partials = nullptr;      // [nExtra][nImg][T * T]
   uint64_t nExtra = 0;
   size_t extraCapacity = 0, partialCapacity = 0;
@@ -173,7 +178,8 @@ struct Plan {
   // (complex kind), [6 + kind] bins used, [8 + kind] occupied tiles,
   // [10 + kind] needed tiles, [12 + kind] bins failing the plan check,
   // [14 + kind] live (layer, tile row) pairs, [16 + kind] (layer, tile
-  // column) pairs holding a needed tile
+  // column) pairs holding a needed tile, [18 + kind] samples listed as not
+  // gridded
   unsigned long long* counters = nullptr;
   uint32_t* queue = nullptr;  // work-queue head of the running grid launch
   // rows sorted by their middle channel's bin: sort keys and row indices
@@ -269,6 +275,9 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
 hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumulate,
                       bool sparseLayers = false);
 hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass);
+// NaN for every sample of a kind that no record grids (with degridBinned
+// over all passes, every value of the predicted visibilities is written).
+hws_status markSkipped(hws_gridder* g, float* dVisOut);
 // Sorts n records (and their 32-bit keys) by key bits [0, endBit) on the
 // context's stream; on return `recs` holds the sorted records (the two record
 // buffers may have been swapped).

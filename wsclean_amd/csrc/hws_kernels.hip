@@ -203,6 +203,9 @@ struct KindArgs {
   unsigned long long* counters;  // gridded, off grid, off layer
   uint32_t* sortKeys;            // scatter: bin << 12 | footprint origin, or null
   uint32_t nRecords;             // scatter: records of the plan
+  uint32_t* skipped;             // scatter: samples no record grids [nSkipped]
+  uint32_t nSkipped;
+  unsigned long long* skippedCursor;
 };
 
 // The uv tiles a sample's footprint touches. Footprint cells run from
@@ -375,6 +378,17 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
         waveCount(&sCount[0], ok);
         waveCount(&sCount[1], inRange && p.layer >= 0 && !p.onGrid);
         waveCount(&sCount[2], inRange && p.layer < 0);
+      } else {
+        // list the samples no record grids (one atomic per wave)
+        const bool skip = inRange && !ok;
+        const unsigned long long sm = __ballot(skip);
+        if (sm) {
+          unsigned long long at0 = 0ull;
+          if (lane == 0) at0 = atomicAdd(k.skippedCursor, (unsigned long long)__popcll(sm));
+          at0 = __shfl(at0, 0);
+          const unsigned long long at = at0 + __popcll(sm & ((1ull << lane) - 1ull));
+          if (skip && at < k.nSkipped) k.skipped[at] = idx;
+        }
       }
       TileSpan sx, sy;
       sx.tA = sx.tB = sy.tA = sy.tB = -1;
@@ -464,6 +478,19 @@ planCheckKernel(const uint32_t* __restrict__ cursor, const uint32_t* __restrict_
   const bool wrong = b < nbins && cursor[b] != binStart[b + 1];
   const int cnt = __syncthreads_count(wrong);
   if (threadIdx.x == 0 && cnt) atomicAdd(bad, (unsigned long long)cnt);
+}
+
+// NaN for the images of one kind of every sample no record of that kind
+// grids (the predict's "not predicted" value; the degridding writes all
+// others).
+__global__ void __launch_bounds__(kThreads)
+nanSkippedKernel(const uint32_t* __restrict__ skipped, size_t n, ImgList imgs, int nImgTotal,
+                 float2* __restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float q = __int_as_float(0x7FC00000);
+  const size_t base = size_t(skipped[i]) * nImgTotal;
+  for (int gi = 0; gi < imgs.n; ++gi) out[base + imgs.idx[gi]] = make_float2(q, q);
 }
 
 // Work items per bin: a bin of c > 0 records is processed as ceil(c / chunk)
@@ -2270,7 +2297,7 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     HWS_TRY(ensureAlloc(g, kp.itemStart, kp.itemCapacity, pl.nbins + 1, "plan item offsets"));
     HWS_TRY(launchFill(g, kp.counts, 0u, pl.nbins + 1));
     KindArgs ka{layerMap(g, kind), kp.counts, nullptr, nullptr, pl.counters + 3 * kind,
-                nullptr, 0u};
+                nullptr, 0u, nullptr, 0u, pl.counters + 18 + kind};
     if (n > 0) {
 #define HWS_COUNT(P)                                                                      \
   hipLaunchKernelGGL((binKernel<false, false, P>), dim3(cBlocks), dim3(kThreads), 0, g->stream, \
@@ -2292,6 +2319,9 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     HWS_TRY(exclusiveScan(g, kp.counts, kp.itemStart, pl.nbins, pl.scanScratch));
     // the record and item totals size the sorted array and partials
     uint32_t totals[3] = {0, 0, 0};
+    unsigned long long placed[3] = {0ull, 0ull, 0ull};
+    HWS_TRY(checkHip(g, hipMemcpyAsync(placed, pl.counters + 3 * kind, sizeof(placed),
+                                       hipMemcpyDeviceToHost, g->stream), "sample counts"));
     HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[0], kp.binStart + pl.nbins, sizeof(uint32_t),
                                        hipMemcpyDeviceToHost, g->stream), "record total"));
     HWS_TRY(checkHip(g, hipMemcpyAsync(&totals[1], kp.extraStart + pl.nbins, sizeof(uint32_t),
@@ -2300,6 +2330,9 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                                        hipMemcpyDeviceToHost, g->stream), "item total"));
     HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "plan totals"));
     const uint32_t total = totals[0];
+    kp.nSkipped = placed[1] + placed[2];
+    HWS_TRY(ensureAlloc(g, kp.skipped, kp.skippedCapacity,
+                        std::max<size_t>(size_t(kp.nSkipped), 1), "plan skipped samples"));
     kp.nItems = total;
     kp.nExtra = totals[1];
     kp.nWorkItems = totals[2];
@@ -2345,6 +2378,8 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     ka.cursor = kp.counts;
     ka.recSorted = kp.recSorted;
     ka.nRecords = total;
+    ka.skipped = kp.skipped;
+    ka.nSkipped = uint32_t(kp.nSkipped);
     // sort keys bin << 12 | origin need 12 + log2(nbins) <= 32 bits
     int binBits = 0;
     while ((size_t(1) << binBits) < pl.nbins) ++binBits;
@@ -2380,11 +2415,14 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
       if (kp.recSorted != r0) std::swap(kp.capacity, kp.altCapacity);
     }
   }
-  unsigned long long bad[2] = {0ull, 0ull};
-  HWS_TRY(checkHip(g, hipMemcpyAsync(bad, pl.counters + 12, sizeof(bad), hipMemcpyDeviceToHost,
+  unsigned long long c[kPlanCounters] = {};
+  HWS_TRY(checkHip(g, hipMemcpyAsync(c, pl.counters, sizeof(c), hipMemcpyDeviceToHost,
                                      g->stream), "plan check"));
   HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "plan check"));
-  if (bad[0] || bad[1])
+  bool listed = true;
+  for (int kind = 0; kind < 2; ++kind)
+    if (pl.kinds[kind].active && c[18 + kind] != pl.kinds[kind].nSkipped) listed = false;
+  if (c[12] || c[13] || !listed)
     return setLastError(g, HWS_ERR_BAD_STATE, "binning plan inconsistent: records misplaced");
   pl.valid = true;
   return HWS_OK;
@@ -2573,6 +2611,24 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
         HWS_TRY(checkHip(g, hipGetLastError(), "partial reduce launch"));
       }
     }
+  }
+  return HWS_OK;
+}
+
+hws_status markSkipped(hws_gridder* g, float* dVisOut) {
+  const Plan& pl = g->plan;
+  if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
+  for (int kind = 0; kind < 2; ++kind) {
+    const KindPlan& kp = pl.kinds[kind];
+    if (!kp.active || kp.nSkipped == 0) continue;
+    ImgList imgs;
+    imgs.n = kp.nImg;
+    for (int i = 0; i < kMaxImages; ++i) imgs.idx[i] = kp.img[i];
+    const size_t n = size_t(kp.nSkipped);
+    hipLaunchKernelGGL(nanSkippedKernel, dim3(unsigned((n + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, g->stream, kp.skipped, n, imgs, g->nImg,
+                       reinterpret_cast<float2*>(dVisOut));
+    HWS_TRY(checkHip(g, hipGetLastError(), "not-predicted launch"));
   }
   return HWS_OK;
 }
