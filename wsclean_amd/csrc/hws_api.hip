@@ -455,6 +455,7 @@ void hws_destroy(hws_gridder* g) {
   freeDev(pl.scanScratch);
   freeDev(pl.counters);
   freeDev(pl.queue);
+  freeDev(pl.rot);
   freeDev(pl.rowKeys);
   freeDev(pl.rowKeysAlt);
   freeDev(pl.rowOrder);

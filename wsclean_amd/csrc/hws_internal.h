@@ -182,6 +182,9 @@ struct Plan {
   // gridded
   unsigned long long* counters = nullptr;
   uint32_t* queue = nullptr;  // work-queue head of the running grid launch
+  // rotating contexts: the gridding phase-centre phasor of every sample
+  float2* rot = nullptr;
+  size_t rotCapacity = 0;
   // rows sorted by their middle channel's bin: sort keys and row indices
   // (and the second buffers of the sort)
   uint32_t* rowKeys = nullptr;

@@ -738,6 +738,7 @@ struct GridArgs {
   int sg;            // degrid subgrid edge = tileSize + K - 1
   size_t nchan;
   uint32_t chunk;    // records per work item
+  const float2* rot;  // rotating contexts: the gridding phasor of every sample
 };
 
 // The phase-centre rotation of a sample as a unit phasor, once per sample
@@ -766,6 +767,17 @@ __device__ __forceinline__ float2 rotationPhasor(const GridArgs& a,
 
 __device__ __forceinline__ float2 rotateBy(float2 v, float2 p) {
   return make_float2(v.x * p.x - v.y * p.y, v.x * p.y + v.y * p.x);
+}
+
+// The gridding phasor of every sample of a plan (rotationPhasor with sign
+// -1), once per plan: the grid and degrid kernels read it per record
+// instead of evaluating the FP64 argument and its sine and cosine for each
+// of a sample's records and image kinds.
+__global__ void __launch_bounds__(256)
+rotationTableKernel(GridArgs a, const double* __restrict__ uvw, const double* __restrict__ wl,
+                    size_t n, float2* __restrict__ rot) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) rot[i] = rotationPhasor(a, uvw, wl, uint32_t(i), -1.0);
 }
 
 // Kernel tables into LDS: FP32 tables, or FP16 (hi, lo) pairs whose sum
@@ -887,8 +899,7 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
           const int kk = int(rec.y >> 12);
           const int ky = kk / OS;
           myPos[lane] = make_int2(int(rec.y & 4095u), (ky * K) | ((kk - ky * OS) * K) << 16);
-          const float2 ph = a.ga.rotate ? rotationPhasor(a, uvw, wl, idx, -1.0)
-                                        : make_float2(1.0f, 0.0f);
+          const float2 ph = a.ga.rotate ? a.rot[idx] : make_float2(1.0f, 0.0f);
 #pragma unroll
           for (int gi = 0; gi < NIMG; ++gi) {
             float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
@@ -1146,7 +1157,7 @@ gridCellsKernel(GridArgs a, const uint32_t* __restrict__ binStart,
           kof = (ky * K) | (kx * K) << 16;
         }
         float2 ph = make_float2(1.0f, 0.0f);
-        if (ROT) ph = rotationPhasor(a, uvw, wl, idx, -1.0);
+        if (ROT) ph = a.rot[idx];
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
           float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
@@ -1346,7 +1357,7 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
         pos = (kx * rowLen + (T - 1) + (K - 1) - xs) |
               ((ky * rowLen + (T - 1) + (K - 1) - ys) << 16);
         float2 ph = make_float2(1.0f, 0.0f);
-        if (ROT) ph = rotationPhasor(a, uvw, wl, idx, -1.0);
+        if (ROT) ph = a.rot[idx];
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
           float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
@@ -1511,7 +1522,13 @@ __device__ __forceinline__ void degridRecord(const GridArgs& a, uint2 rec,
       }
     }
   }
-  const float2 ph = a.ga.rotate ? rotationPhasor(a, uvw, wl, idx, +1.0) : make_float2(1.0f, 0.0f);
+  // the degridding phasor is the conjugate of the gridding one (bit for
+  // bit: the argument and both sincos forms are odd in it)
+  float2 ph = make_float2(1.0f, 0.0f);
+  if (a.ga.rotate) {
+    ph = a.rot[idx];
+    ph.y = -ph.y;
+  }
 #pragma unroll
   for (int gi = 0; gi < NIMG; ++gi) {
     // folded samples return the sample, others its conjugate (:681-684)
@@ -2424,6 +2441,15 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     if (pl.kinds[kind].active && c[18 + kind] != pl.kinds[kind].nSkipped) listed = false;
   if (c[12] || c[13] || !listed)
     return setLastError(g, HWS_ERR_BAD_STATE, "binning plan inconsistent: records misplaced");
+  if (ga.rotate && n > 0) {
+    HWS_TRY(ensureAlloc(g, pl.rot, pl.rotCapacity, n, "plan rotation phasors"));
+    GridArgs ra{};
+    ra.ga = ga;
+    ra.nchan = nchan;
+    hipLaunchKernelGGL(rotationTableKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                       g->stream, ra, dUvw, dWl, n, pl.rot);
+    HWS_TRY(checkHip(g, hipGetLastError(), "rotation table launch"));
+  }
   pl.valid = true;
   return HWS_OK;
 }
@@ -2442,6 +2468,7 @@ GridArgs gridArgs(const hws_gridder* g, int pass) {
   a.sg = pl.tileSize + g->K - 1;
   a.nchan = pl.nchan;
   a.chunk = pl.chunk;
+  a.rot = pl.rot;
   return a;
 }
 }  // namespace
