@@ -782,7 +782,9 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
   (void)hipSetDevice(g->cfg.device);
   Plan& pl = g->plan;
   HWS_TRY(record(g, 2));
-  HWS_TRY(zeroAccumulators(g));
+  // the batched forms' first projection writes the accumulators outright
+  const bool plainForm = g->cfg.precision == HWS_PRECISION_PLAIN_FP32;
+  if (plainForm) HWS_TRY(zeroAccumulators(g));
   float ms[3] = {0, 0, 0};
   for (int pass = 0; pass < g->nPasses; ++pass) {
     if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32) {
@@ -799,7 +801,7 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
     HWS_TRY(fftPassLayers(g, pass, +1,
                           g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && sparseInvert(g)));
     HWS_TRY(record(g, 4));
-    HWS_TRY(projectPass(g, pass));
+    HWS_TRY(projectPass(g, pass, !plainForm && pass == 0));
     HWS_TRY(record(g, 5));
     if (g->nPasses > 1) {
       float a = 0, b = 0, c = 0;

@@ -294,7 +294,10 @@ hws_status gridPlain(hws_gridder* g, size_t nrow, size_t nchan,
 hws_status degridPlain(hws_gridder* g, size_t nrow, size_t nchan,
                        const double* dUvw, const double* dWl, float* dVisOut,
                        int pass);
-hws_status projectPass(hws_gridder* g, int pass);
+// fresh: the image accumulators start from zero instead of being read (the
+// first pass of a device invert, which then needs no clear); ignored by the
+// plain form.
+hws_status projectPass(hws_gridder* g, int pass, bool fresh = false);
 hws_status finalizeImages(hws_gridder* g, double mult, float* dReal,
                           float* dImag);
 hws_status initPrediction(hws_gridder* g, const float* dReal,

@@ -1823,6 +1823,7 @@ struct ImgArgs {
   int isComplex[kMaxImages];
   int passStart, nLayersInPass, nLayersPass;
   int blocked;
+  int fresh;  // the accumulators start from zero (not read)
 };
 
 // n' = sqrt(1 - l^2 - m^2) - 1 inside the horizon, else 0
@@ -1857,11 +1858,14 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
   double np = nPrime(l, m);
   if (a.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
   const size_t o = size_t(img) * HW + size_t(ySrc) * a.W + xSrc;
-  float s = sum[o], c = comp[o];
-  float is = 0.0f, ic = 0.0f;
-  if (cplx) {
-    is = isum[o];
-    ic = icomp[o];
+  float s = 0.0f, c = 0.0f, is = 0.0f, ic = 0.0f;
+  if (!a.fresh) {
+    s = sum[o];
+    c = comp[o];
+    if (cplx) {
+      is = isum[o];
+      ic = icomp[o];
+    }
   }
   const LayerMap& map = a.maps[cplx];
   const float2* src = layers + size_t(img) * a.nLayersPass * HW + p;
@@ -1888,7 +1892,8 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
   }
   sum[o] = s;
   comp[o] = c;
-  if (cplx) {
+  // (a fresh pass also zeroes the imaginary accumulators of real images)
+  if (cplx || (a.fresh && isum)) {
     isum[o] = is;
     icomp[o] = ic;
   }
@@ -2155,6 +2160,7 @@ ImgArgs imgArgs(const hws_gridder* g, int pass) {
   a.nLayersInPass = passLayerStart(g, pass + 1) - a.passStart;
   a.nLayersPass = g->maxLayersPerPass;
   a.blocked = g->blocked;
+  a.fresh = 0;
   return a;
 }
 
@@ -2780,9 +2786,10 @@ hws_status ensureNTable(hws_gridder* g, int order) {
 }
 }  // namespace
 
-hws_status projectPass(hws_gridder* g, int pass) {
+hws_status projectPass(hws_gridder* g, int pass, bool fresh) {
   const size_t HW = size_t(g->W) * g->H;
-  const ImgArgs a = imgArgs(g, pass);
+  ImgArgs a = imgArgs(g, pass);
+  a.fresh = fresh ? 1 : 0;
   if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32) {
     HWS_TRY(ensureNTable(g, 0));
     for (int img = 0; img < g->nImg; ++img) {
