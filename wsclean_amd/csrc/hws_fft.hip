@@ -320,8 +320,12 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
 }
 
 // Layers per workgroup of the model-fed rows pass: the workgroup reads the
-// model rows of its row block once from memory and then from the L2.
-constexpr int kModelLayers = 4;
+// model rows of its row block once from memory and then from the L2 (the
+// reference precision's FP64 sincos leaves no registers for that: one).
+template <bool REF>
+constexpr int modelLayers() {
+  return REF ? 1 : 4;
+}
 
 // A store the pass does not read back: marked non-temporal, so the stream of
 // layer writes does not evict what the caches hold (the model image the
@@ -428,8 +432,8 @@ fftRowsFromModelKernel(float2* __restrict__ data, int H, int nbatch,
                        const float2* __restrict__ tw, ModelSource ms) {
   __shared__ float2 lds[4 * N + (4 * N) / 64];
 #pragma unroll 1
-  for (int li = 0; li < kModelLayers; ++li) {
-    const int bi = int(blockIdx.y) * kModelLayers + li;
+  for (int li = 0; li < modelLayers<REF>(); ++li) {
+    const int bi = int(blockIdx.y) * modelLayers<REF>() + li;
     if (bi >= nbatch) break;  // workgroup-uniform
     // the empty asm gives the model pointers, pixel scale, twiddle table and
     // thread index new values each iteration: nothing derived from them is
@@ -969,7 +973,9 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     } else if (sign < 0 && sparse) {
       if (model) {
         ok = false;
-        const dim3 mgrid(H / 4, (nbatch + kModelLayers - 1) / kModelLayers);
+        const int ml = model->precision == HWS_PRECISION_REFERENCE ? modelLayers<true>()
+                                                                   : modelLayers<false>();
+        const dim3 mgrid(H / 4, (nbatch + ml - 1) / ml);
         switch (W) {
 #define HWS_MODEL_ROWS(NN)                                                                      \
   case NN:                                                                                      \
