@@ -153,13 +153,15 @@ def run_form(wl_, precision, steps, warmup, tile, device, chunk=0, cycle=True):
     return elapsed / steps, npass, phases, stats_inv, stats_pred, ms_cycle
 
 
-def cpu_baseline(scale_rows=64, threads=None):
+def cpu_baseline(scale_rows=2, threads=None):
     """The oracle's C++ restatement of WSMSGridder (float, layer-routed
     worker lanes, per-thread FFTs) on a bounded sample: Stokes I at 4096^2
     with 64 w-layers, invert + predict, on every n-th row of the workload.
     The fixed per-image cost (FFTs, w-correction) is measured with one row;
     the per-visibility cost from the sample; the full four-image workload
-    time is 4 * fixed + per_vis * N_vis."""
+    time is 4 * fixed + per_vis * N_vis. The sample's per-visibility part
+    must stand well above the run-to-run variation of the fixed part (about
+    a second of 16 on a shared host): every 2nd row gives about 10 s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     from wsclean_amd import synth
@@ -189,8 +191,8 @@ def cpu_baseline(scale_rows=64, threads=None):
     return {
         "value": nvis_full / t_full, "unit": "visibilities/s", "cores": threads, "kind": "port",
         "sample": (f"oracle C++ restatement of WSMSGridder (float, {threads} threads), Stokes I "
-                   f"4096^2 nW=64 invert+predict: 1 row (fixed cost {t_fixed:.2f} s) and every "
-                   f"{scale_rows}th row ({ns} vis, {t_sample:.2f} s); full 4-correlation time "
+                   f"4096^2 nW=64 invert+predict: 1 row (fixed cost {t_fixed:.2f} s) and one row "
+                   f"in {scale_rows} ({ns} vis, {t_sample:.2f} s); full 4-correlation time "
                    f"modelled as 4*fixed + per-vis*{nvis_full}"),
         "seconds": t_fixed + t_sample,
     }
