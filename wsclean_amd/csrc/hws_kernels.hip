@@ -1620,62 +1620,78 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
 }
 
 
+constexpr int kReduceParts = 4;  // threads summing one cell's partial tiles
+
 // Adds the partial tiles of the extra items of every split bin of a pass to
-// the layer, in item order (a fixed summation order). Task (e, group): if e
-// is the first extra item of its bin, the workgroup sums that bin's partial
-// tiles for one group of blockDim cells, one cell per thread (coalesced over
-// the cells of each partial tile).
+// the layer, in a fixed summation order. Task (e, group): if e is the first
+// extra item of its bin, the workgroup sums that bin's partial tiles for one
+// group of 64 cells, four threads per cell (coalesced over the cells of each
+// partial tile): the dense uv centre's bins hold thousands of extra items,
+// and one thread per cell left their sums latency-bound.
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
 reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
                      const float2* __restrict__ partials, float2* __restrict__ layers) {
+  __shared__ float2 red[kReduceParts][kThreads / kReduceParts];
   const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
   const uint32_t eLo = extraStart[binLo], eHi = extraStart[binHi];
   const int T = a.ga.tileSize, T2 = T * T;
-  const uint32_t groups = uint32_t((T2 + blockDim.x - 1) / blockDim.x);
+  constexpr int kCells = kThreads / kReduceParts;  // cells per task
+  const uint32_t groups = uint32_t((T2 + kCells - 1) / kCells);
   const size_t HW = size_t(a.ga.W) * a.ga.H;
   const size_t tasks = size_t(eHi - eLo) * groups;
+  const int part = int(threadIdx.x) / kCells, cl = int(threadIdx.x) - part * kCells;
   for (size_t task = blockIdx.x; task < tasks; task += gridDim.x) {
     const uint32_t e = eLo + uint32_t(task / groups);
     const uint32_t grp = uint32_t(task - size_t(e - eLo) * groups);
     const uint32_t bin = ownerOf(extraStart, binLo, binHi, e);
     const uint32_t first = extraStart[bin];
-    if (e != first) continue;  // another task of the bin does the work
-    const int c = int(grp * blockDim.x + threadIdx.x);
-    if (c >= T2) continue;
+    if (e != first) continue;  // another task of the bin does the work (workgroup-uniform)
+    const int c = int(grp) * kCells + cl;
+    const bool valid = c < T2;
     const uint32_t nEx = extraStart[bin + 1] - first;
     const int layer = int(bin / uint32_t(a.tilesPerLayer));
     const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
     const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
     const int cy = c / T, cx = c - cy * T;
-    const size_t off = cellIndex(tx * T + cx, ty * T + cy, a.ga.W, a.ga.blocked);
 #pragma unroll
     for (int gi = 0; gi < NIMG; ++gi) {
-      const float2* src = partials + (size_t(first) * NIMG + gi) * T2 + c;
-      const size_t stride = size_t(NIMG) * T2;
-      // four independent partial sums keep four loads in flight
+      // part p of a cell sums the partial tiles p, p + 4, p + 8, ... with
+      // four independent sums (sixteen loads of a cell in flight); the
+      // parts are then added in a fixed order
       float2 s4[4] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
-      uint32_t q = 0;
-      for (; q + 4 <= nEx; q += 4) {
+      if (valid) {
+        const float2* src = partials + (size_t(first) * NIMG + gi) * T2 + c;
+        const size_t stride = size_t(NIMG) * T2;
+        uint32_t q = uint32_t(part);
+        for (; q + 3u * kReduceParts < nEx; q += 4u * kReduceParts) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const float2 v = src[size_t(q + k) * stride];
-          s4[k].x += v.x;
-          s4[k].y += v.y;
+          for (int k = 0; k < 4; ++k) {
+            const float2 v = src[size_t(q + uint32_t(k * kReduceParts)) * stride];
+            s4[k].x += v.x;
+            s4[k].y += v.y;
+          }
+        }
+        for (; q < nEx; q += kReduceParts) {
+          const float2 v = src[size_t(q) * stride];
+          s4[0].x += v.x;
+          s4[0].y += v.y;
         }
       }
-      for (; q < nEx; ++q) {
-        const float2 v = src[size_t(q) * stride];
-        s4[0].x += v.x;
-        s4[0].y += v.y;
+      red[part][cl] = make_float2((s4[0].x + s4[1].x) + (s4[2].x + s4[3].x),
+                                  (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y));
+      __syncthreads();
+      if (part == 0 && valid) {
+        const float2 r0 = red[0][cl], r1 = red[1][cl], r2 = red[2][cl], r3 = red[3][cl];
+        const size_t off = cellIndex(tx * T + cx, ty * T + cy, a.ga.W, a.ga.blocked);
+        float2* dst = layers +
+                      (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
+        const float2 o = *dst;
+        *dst = make_float2(o.x + ((r0.x + r1.x) + (r2.x + r3.x)),
+                           o.y + ((r0.y + r1.y) + (r2.y + r3.y)));
       }
-      const float sx = (s4[0].x + s4[1].x) + (s4[2].x + s4[3].x);
-      const float sy = (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y);
-      float2* dst = layers +
-                    (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
-      const float2 o = *dst;
-      *dst = make_float2(o.x + sx, o.y + sy);
+      __syncthreads();  // red is rewritten by the next image or task
     }
   }
 }
@@ -2627,7 +2643,8 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
       }
       HWS_TRY(checkHip(g, hipGetLastError(), "grid launch"));
       if (kp.nExtra) {
-        const uint64_t groups = (uint64_t(pl.tileSize) * pl.tileSize + kThreads - 1) / kThreads;
+        const uint64_t cells = kThreads / kReduceParts;
+        const uint64_t groups = (uint64_t(pl.tileSize) * pl.tileSize + cells - 1) / cells;
         const unsigned rblocks = unsigned(std::min<uint64_t>(kp.nExtra * groups, 65536));
         switch (n) {
 #define HWS_REDUCE_CASE(N)                                                                    \
