@@ -1625,9 +1625,10 @@ constexpr int kReduceParts = 4;  // threads summing one cell's partial tiles
 // Adds the partial tiles of the extra items of every split bin of a pass to
 // the layer, in a fixed summation order. Task (e, group): if e is the first
 // extra item of its bin, the workgroup sums that bin's partial tiles for one
-// group of 64 cells, four threads per cell (coalesced over the cells of each
-// partial tile): the dense uv centre's bins hold thousands of extra items,
-// and one thread per cell left their sums latency-bound.
+// group of 256 / kReduceParts cells, kReduceParts threads per cell
+// (coalesced over the cells of each partial tile): the dense uv centre's
+// bins hold thousands of extra items, and one thread per cell left their
+// sums latency-bound.
 template <int NIMG>
 __global__ void __launch_bounds__(kThreads)
 reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
@@ -1657,9 +1658,9 @@ reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
     const int cy = c / T, cx = c - cy * T;
 #pragma unroll
     for (int gi = 0; gi < NIMG; ++gi) {
-      // part p of a cell sums the partial tiles p, p + 4, p + 8, ... with
-      // four independent sums (sixteen loads of a cell in flight); the
-      // parts are then added in a fixed order
+      // part p of a cell sums the partial tiles p, p + P, p + 2P, ... (P
+      // parts) with four independent sums; the parts are then added in
+      // part order
       float2 s4[4] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
       if (valid) {
         const float2* src = partials + (size_t(first) * NIMG + gi) * T2 + c;
@@ -1683,13 +1684,17 @@ reducePartialsKernel(GridArgs a, const uint32_t* __restrict__ extraStart,
                                   (s4[0].y + s4[1].y) + (s4[2].y + s4[3].y));
       __syncthreads();
       if (part == 0 && valid) {
-        const float2 r0 = red[0][cl], r1 = red[1][cl], r2 = red[2][cl], r3 = red[3][cl];
+        float2 t = red[0][cl];
+#pragma unroll
+        for (int pp = 1; pp < kReduceParts; ++pp) {
+          t.x += red[pp][cl].x;
+          t.y += red[pp][cl].y;
+        }
         const size_t off = cellIndex(tx * T + cx, ty * T + cy, a.ga.W, a.ga.blocked);
         float2* dst = layers +
                       (size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW + off;
         const float2 o = *dst;
-        *dst = make_float2(o.x + ((r0.x + r1.x) + (r2.x + r3.x)),
-                           o.y + ((r0.y + r1.y) + (r2.y + r3.y)));
+        *dst = make_float2(o.x + t.x, o.y + t.y);
       }
       __syncthreads();  // red is rewritten by the next image or task
     }
