@@ -511,9 +511,11 @@ fftColsBlockedSparseKernel(float2* __restrict__ data, int W, const float2* __res
                            int tshift) {
   __shared__ float2 lds[4 * N + (4 * N) / 64];
   __shared__ unsigned char live[N / 4];
-  const int nb = gridDim.x;
-  int cb = blockIdx.x;
-  if ((nb & 7) == 0) cb = (cb & 7) * (nb >> 3) + (cb >> 3);
+  // column blocks in their natural order, so consecutive workgroups (dealt
+  // round-robin to the XCDs) spread over all XCDs: the column blocks with
+  // needed tiles lie near the uv origin, at both ends of the row, and a
+  // contiguous range per XCD would leave most XCDs without work
+  const int cb = blockIdx.x;
   const int tilesX = W >> tshift;
   const int tx = (cb * 4) >> tshift;
   const size_t layerBase = size_t(layer0 + int(blockIdx.y)) * tilesY;
@@ -694,7 +696,9 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   int blk = blockIdx.x;
-  if ((nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
+  // (the sparse forward mode keeps the natural order: its live column blocks
+  // lie near the uv origin, at both ends of the row)
+  if (MODE != 2 && (nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
   const int pitch = H + 1;
   const int layer = sp.layer0 + int(blockIdx.y);
   const int tx = (blk * 4) >> sp.tshift;
