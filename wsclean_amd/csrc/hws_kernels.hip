@@ -304,14 +304,21 @@ constexpr uint32_t kNoBin = 0xFFFFFFFFu;
 // Slot of `bin` in a workgroup's LDS table, inserted if absent (linear
 // probing from a multiplicative hash); -1 when 16 probes find neither the bin
 // nor a free slot (the caller then counts in memory directly).
-__device__ __forceinline__ int binSlot(uint32_t* keys, uint32_t bin) {
+// The inserting thread appends the slot to `used` (count in *nUsed), so the
+// table's flush and reset visit the used slots only.
+__device__ __forceinline__ int binSlot(uint32_t* keys, uint32_t bin, uint16_t* used,
+                                       uint32_t* nUsed) {
   uint32_t h = (bin * 2654435761u) >> 22;
   for (int probe = 0; probe < 16; ++probe) {
     const uint32_t k = *reinterpret_cast<volatile uint32_t*>(&keys[h]);
     if (k == bin) return int(h);
     if (k == kNoBin) {
       const uint32_t old = atomicCAS(&keys[h], kNoBin, bin);
-      if (old == kNoBin || old == bin) return int(h);
+      if (old == kNoBin) {
+        used[atomicAdd(nUsed, 1u)] = uint16_t(h);
+        return int(h);
+      }
+      if (old == bin) return int(h);
     }
     h = (h + 1u) & uint32_t(kBinSlots - 1);
   }
@@ -339,16 +346,20 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
           uint32_t step, GeomArgs ga, KindArgs k) {
   __shared__ uint32_t slotBin[kBinSlots];
   __shared__ uint32_t slotCount[kBinSlots];  // scatter: then the bin's first position
+  __shared__ uint16_t used[kBinSlots];       // the group's used slots
+  __shared__ uint32_t nUsed[2];              // by group parity (reset one group later)
   __shared__ unsigned long long sCount[3];
   for (int t = threadIdx.x; t < kBinSlots; t += blockDim.x) {
     slotBin[t] = kNoBin;
     slotCount[t] = 0u;
   }
+  if (threadIdx.x < 2) nUsed[threadIdx.x] = 0u;
   if (!SCATTER && threadIdx.x < 3) sCount[threadIdx.x] = 0ull;
   __syncthreads();
   const uint32_t n = uint32_t(nrow * size_t(nchan));
   const int lane = threadIdx.x & 63;
-  for (uint32_t q = blockIdx.x; q < ngroups; q += gridDim.x) {
+  uint32_t parity = 0u;
+  for (uint32_t q = blockIdx.x; q < ngroups; q += gridDim.x, parity ^= 1u) {
     const uint32_t grp = uint32_t((uint64_t(q) * step) % ngroups);
     // per sample and emission: the record, and on the run head its table
     // slot and offset (or, past a full table, its position in memory)
@@ -403,7 +414,7 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
         uint32_t m = has ? 1u : 0u;
         uint32_t l = 0u;
         if (has && run.head == lane) {
-          const int slot = binSlot(slotBin, bin);
+          const int slot = binSlot(slotBin, bin, used, &nUsed[parity]);
           if (slot >= 0) {
             l = atomicAdd(&slotCount[slot], uint32_t(run.len));
             m |= uint32_t(slot) << 8;
@@ -423,10 +434,11 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
       }
     }
     __syncthreads();
-    // one count per distinct bin of the group
-    for (int t = threadIdx.x; t < kBinSlots; t += blockDim.x) {
+    // one count per distinct bin of the group (the used slots only)
+    const uint32_t nu = nUsed[parity];
+    for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) {
+      const int t = used[u];
       const uint32_t bin = slotBin[t];
-      if (bin == kNoBin) continue;
       if (!SCATTER) {
         atomicAdd(&k.counts[bin], slotCount[t]);
         slotBin[t] = kNoBin;
@@ -435,6 +447,9 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
         slotCount[t] = atomicAdd(&k.cursor[bin], slotCount[t]);
       }
     }
+    // every thread has read nUsed[parity]; the next group counts in the
+    // other one, and this one is cleared before the group after
+    if (threadIdx.x == 0) nUsed[parity ^ 1u] = 0u;
     __syncthreads();
     if (SCATTER) {
 #pragma unroll
@@ -455,7 +470,8 @@ binKernel(const double* __restrict__ uvw, const double* __restrict__ wl,
           }
         }
       __syncthreads();
-      for (int t = threadIdx.x; t < kBinSlots; t += blockDim.x) {
+      for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) {
+        const int t = used[u];
         slotBin[t] = kNoBin;
         slotCount[t] = 0u;
       }
