@@ -501,3 +501,50 @@ def test_tile_sizes_match_oracle(hws, oracle, tile):
     assert np.array_equal(np.isnan(v), ~ok)
     assert rel(v[ok], v_o[ok]) < TOL_IMG["mixed"]
     d.close()
+
+
+@pytest.mark.parametrize("precision", ["reference", "mixed"])
+def test_complex_only_with_skipped_samples(hws, oracle, precision):
+    # only a complex image (the real-image kind is absent, so the rows are
+    # ordered by the complex kind's layer map), uv reaching past the grid
+    # edge and w past the layer range: a large share of the samples is
+    # gridded by no record; the invert must match the oracle, and the
+    # predict must return NaN exactly for those samples (listed by the plan)
+    import torch
+    w = 128
+    uvw, wl, vis, dl = case(77, w, nrow=1200, nchan=4, frac=0.7, nimg=1)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = 0.7 * np.abs(uvw_l[:, 2]).max()
+    nw = 6
+    (img_r, img_i), _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis[:, :, 0].reshape(-1), nw,
+                                      0.0, maxw, cplx=True)
+    dev = torch.device("cuda:0")
+    d = hws.DeviceGridder(w, w, dl, dl, 7, 63, precision=precision, n_images=1,
+                          complex_images=(0,))
+    d.prepare_w_layers(nw, 0.0, maxw)
+    d_uvw, d_wl = torch.from_numpy(uvw).to(dev), torch.from_numpy(wl).to(dev)
+    d_vis = torch.from_numpy(vis).to(dev)
+    out = torch.zeros((1, w, w), dtype=torch.float32, device=dev)
+    out_i = torch.zeros_like(out)
+    d.plan(d_uvw, d_wl)
+    d.invert(d_vis, 1.0, out, out_i)
+    st = d.stats()
+    skipped = st["samples_off_grid"] + st["samples_off_layer"]
+    assert st["samples_off_grid"] > 0 and st["samples_off_layer"] > 0
+    assert st["samples_gridded"] + skipped == uvw_l.shape[0]
+    assert st["tiles_occupied"][0] == 0 and st["tiles_occupied"][1] > 0
+    assert rel(out.cpu().numpy()[0], img_r) < TOL_IMG[precision]
+    assert rel(out_i.cpu().numpy()[0], img_i) < TOL_IMG[precision]
+    rng = np.random.default_rng(78)
+    mre = rng.standard_normal((1, w, w)).astype(np.float32)
+    mim = rng.standard_normal((1, w, w)).astype(np.float32)
+    # the output starts as a finite value everywhere: every entry must be
+    # written, NaN where no record grids the sample
+    pred = torch.full_like(d_vis, 7.0 + 7.0j)
+    d.predict(torch.from_numpy(mre).to(dev), torch.from_numpy(mim).to(dev), pred)
+    v_o = oracle_predict(oracle, w, w, dl, dl, uvw_l, mre[0], mim[0], nw, 0.0, maxw, cplx=True)
+    v = pred.cpu().numpy()[:, :, 0].reshape(-1)
+    ok = ~np.isnan(v_o)
+    assert int((~ok).sum()) == skipped
+    assert np.array_equal(np.isnan(v), ~ok)
+    assert rel(v[ok], v_o[ok]) < TOL_IMG[precision]
