@@ -336,6 +336,13 @@ __device__ __forceinline__ void streamStore(float2* p, float2 v) {
   __builtin_nontemporal_store(x, reinterpret_cast<f2*>(p));
 }
 
+// A load of a layer element the pass reads once (non-temporal).
+__device__ __forceinline__ float2 streamLoad(const float2* p) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 v = __builtin_nontemporal_load(reinterpret_cast<const f2*>(p));
+  return make_float2(v.x, v.y);
+}
+
 // MODEL (rows pass of a forward transform): the first stage computes its
 // inputs from the model image (ms, w; y0 = the row block's first row)
 // instead of reading the layer.
@@ -365,8 +372,13 @@ __device__ __forceinline__ void blockedStage(float2* lds, float2* __restrict__ g
       if (FIRST && MODEL)
         x = modelCell<MODEL == 2>(*ms, w, pos, y0 + f, W, H);
       else if (FIRST)
-        x = (live && liveIn && !live[pos >> tshift]) ? make_float2(0.0f, 0.0f)
-                                                    : g[blockedAddr<ROWS>(pos, f, W)];
+        // (the forward columns pass reads the rows pass's output once: a
+        // non-temporal load; measured slower in the backward passes, whose
+        // inputs the caches partly hold)
+        x = (live && liveIn && !live[pos >> tshift])
+                ? make_float2(0.0f, 0.0f)
+                : (SIGN < 0 && !ROWS ? streamLoad(&g[blockedAddr<ROWS>(pos, f, W)])
+                                     : g[blockedAddr<ROWS>(pos, f, W)]);
       else
         x = lds[padIdx(pos * 4 + f)];
       if (!FIRST && r != 0) x = cmul(x, tw[jm * r * TWSTEP]);

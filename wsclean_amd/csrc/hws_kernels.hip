@@ -1907,7 +1907,10 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
   const LayerMap& map = a.maps[cplx];
   const float2* src = layers + size_t(img) * a.nLayersPass * HW + p;
   for (int li = 0; li < a.nLayersInPass; ++li) {
-    const float2 F = src[size_t(li) * HW];
+    // (streamed once: a non-temporal load)
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 fv = __builtin_nontemporal_load(reinterpret_cast<const f2*>(src + size_t(li) * HW));
+    const float2 F = make_float2(fv.x, fv.y);
     const double w = layerToW(map, a.passStart + li);
     float re, im;
     if (a.precision == HWS_PRECISION_REFERENCE) {
