@@ -369,4 +369,32 @@ void orc_fft2d(size_t width, size_t height, int sign, double* data) {
   fft.Transform(reinterpret_cast<std::complex<double>*>(data));
 }
 
+void orc_calculate_w_limits(const orc_ms_settings* s, size_t nrow, size_t nchan,
+                            const double* uvw_m, const double* wavelengths,
+                            const float* weights, void* iw, double* min_w,
+                            double* max_w, double* max_baseline_uvw) {
+  GridderSettings gs = toSettings(s);
+  MsView ms;
+  ms.nrow = nrow;
+  ms.nchan = nchan;
+  ms.uvw_m = uvw_m;
+  ms.wavelengths = wavelengths;
+  ms.weights = weights;
+  CalculateWLimits(gs, ms, *static_cast<ImageWeightsRef*>(iw), *min_w, *max_w,
+                   *max_baseline_uvw);
+}
+
+void orc_trim_image(const float* in, size_t w, size_t h, size_t tw, size_t th,
+                    float* out) {
+  std::vector<float> v(in, in + w * h);
+  std::vector<float> r = trimImage(v, w, h, tw, th);
+  std::memcpy(out, r.data(), r.size() * sizeof(float));
+}
+
+void orc_untrim_image(const float* in, size_t tw, size_t th, size_t w, size_t h,
+                      float* out) {
+  std::vector<float> r = untrimImage(in, tw, th, w, h);
+  std::memcpy(out, r.data(), r.size() * sizeof(float));
+}
+
 }  // extern "C"

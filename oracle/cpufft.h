@@ -39,6 +39,18 @@ class Fft1D {
       long double a = sign * 2.0L * pi * (long double)k / (long double)n;
       tw_[k] = std::complex<T>((T)std::cos(a), (T)std::sin(a));
     }
+    // per-stage copies of the twiddles a stage uses, in the order its loop
+    // reads them: entry jm * (radix - 1) + (r - 1) is tw_[jm * r * n / (ns * radix)]
+    size_t ns = 1;
+    for (size_t radix : radices_) {
+      const size_t twStep = n / (ns * radix);
+      std::vector<std::complex<T>> st(ns * (radix - 1));
+      for (size_t jm = 0; jm != ns; ++jm)
+        for (size_t r = 1; r != radix; ++r)
+          st[jm * (radix - 1) + (r - 1)] = tw_[(jm * r * twStep) % n];
+      stageTw_.push_back(std::move(st));
+      ns *= radix;
+    }
   }
 
   size_t Size() const { return n_; }
@@ -50,19 +62,25 @@ class Fft1D {
     size_t ns = 1;
     const size_t n = n_;
     std::complex<T> v[64];
-    for (size_t radix : radices_) {
+    for (size_t s = 0; s != radices_.size(); ++s) {
+      const size_t radix = radices_[s];
       const size_t nr = n / radix;
-      const size_t twStep = n / (ns * radix);
-      for (size_t j = 0; j != nr; ++j) {
-        const size_t jm = j % ns;
-        for (size_t r = 0; r != radix; ++r) {
-          std::complex<T> x = in[j + r * nr];
-          if (r != 0 && jm != 0) x *= tw_[(jm * r * twStep) % n];
-          v[r] = x;
+      const std::complex<T>* tw = stageTw_[s].data();
+      // j = q * ns + jm runs over [0, nr) as in the Stockham recurrence
+      for (size_t q = 0; q != nr / ns; ++q) {
+        for (size_t jm = 0; jm != ns; ++jm) {
+          const size_t j = q * ns + jm;
+          v[0] = in[j];
+          if (jm != 0) {
+            const std::complex<T>* t = tw + jm * (radix - 1) - 1;
+            for (size_t r = 1; r != radix; ++r) v[r] = in[j + r * nr] * t[r];
+          } else {
+            for (size_t r = 1; r != radix; ++r) v[r] = in[j + r * nr];
+          }
+          butterfly(v, radix);
+          const size_t idxD = q * ns * radix + jm;
+          for (size_t r = 0; r != radix; ++r) out[idxD + r * ns] = v[r];
         }
-        butterfly(v, radix);
-        const size_t idxD = (j / ns) * ns * radix + jm;
-        for (size_t r = 0; r != radix; ++r) out[idxD + r * ns] = v[r];
       }
       std::swap(in, out);
       ns *= radix;
@@ -105,6 +123,7 @@ class Fft1D {
   int sign_;
   std::vector<size_t> radices_;
   std::vector<std::complex<T>> tw_;
+  std::vector<std::vector<std::complex<T>>> stageTw_;
 };
 
 // Unnormalised 2-D transform of a height x width row-major array (x fastest),
@@ -119,21 +138,24 @@ class Fft2D {
     std::vector<std::complex<T>> scratch(std::max(w_, h_) * kColBlock);
     for (size_t y = 0; y != h_; ++y)
       rowFft_.Transform(data + y * w_, scratch.data());
-    // columns, kColBlock at a time through a contiguous buffer
-    std::vector<std::complex<T>> col(h_ * kColBlock);
+    // columns, kColBlock at a time through a contiguous buffer whose column
+    // pitch is padded off a power of two (a 2^k pitch puts every element of a
+    // gathered row into one cache set)
+    const size_t pitch = h_ + 8;
+    std::vector<std::complex<T>> col(pitch * kColBlock);
     for (size_t x0 = 0; x0 < w_; x0 += kColBlock) {
       const size_t nb = std::min(kColBlock, w_ - x0);
       for (size_t y = 0; y != h_; ++y)
-        for (size_t b = 0; b != nb; ++b) col[b * h_ + y] = data[y * w_ + x0 + b];
+        for (size_t b = 0; b != nb; ++b) col[b * pitch + y] = data[y * w_ + x0 + b];
       for (size_t b = 0; b != nb; ++b)
-        colFft_.Transform(col.data() + b * h_, scratch.data());
+        colFft_.Transform(col.data() + b * pitch, scratch.data());
       for (size_t y = 0; y != h_; ++y)
-        for (size_t b = 0; b != nb; ++b) data[y * w_ + x0 + b] = col[b * h_ + y];
+        for (size_t b = 0; b != nb; ++b) data[y * w_ + x0 + b] = col[b * pitch + y];
     }
   }
 
  private:
-  static constexpr size_t kColBlock = 8;
+  static constexpr size_t kColBlock = 16;
   size_t w_, h_;
   Fft1D<T> rowFft_, colFft_;
 };

@@ -103,6 +103,8 @@ double longestWavelength(const MsView& ms) {
   return *std::max_element(ms.wavelengths, ms.wavelengths + ms.nchan);
 }
 
+}  // namespace
+
 // aocommon::Image::Trim / Untrim keep the centre of the image.
 std::vector<float> trimImage(const std::vector<float>& in, size_t w, size_t h,
                              size_t tw, size_t th) {
@@ -120,7 +122,6 @@ std::vector<float> untrimImage(const float* in, size_t tw, size_t th, size_t w,
     for (size_t x = 0; x != tw; ++x) out[(x + sx) + (y + sy) * w] = in[x + y * tw];
   return out;
 }
-}  // namespace
 
 // ---------------------------------------------------------------- ImageWeights
 ImageWeightsRef::ImageWeightsRef(WeightingMode mode, double briggsRobustness,

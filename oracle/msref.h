@@ -105,6 +105,15 @@ void CalculateWLimits(const GridderSettings& s, const MsView& ms,
                       const ImageWeightsRef& w, double& minW, double& maxW,
                       double& maxBaselineUVW);
 
+// Centre-keeping trim of a w x h image to tw x th, and its inverse with zero
+// fill (aocommon::Image::Trim/Untrim as WSMSGridder uses them,
+// gridding/wsmsgridder.cpp:476-484, 516-520): trimmed pixel (x, y) is pixel
+// (x + (w - tw)/2, y + (h - th)/2) of the full image.
+std::vector<float> trimImage(const std::vector<float>& in, size_t w, size_t h,
+                             size_t tw, size_t th);
+std::vector<float> untrimImage(const float* in, size_t tw, size_t th, size_t w,
+                               size_t h);
+
 // wsmsgridder.cpp:77-128
 size_t SuggestedWGridSize(const GridderSettings& s, double minW, double maxW);
 

@@ -151,6 +151,20 @@ void orc_dft_predict(size_t width, size_t height, double dl, double dm,
 /* ---- FFT (for tests of the FFT itself) ---- */
 void orc_fft2d(size_t width, size_t height, int sign, double* data);
 
+/* msgridderbase.cpp:200-286: min/max |w| (lambda) over the samples with a
+ * non-zero visibility weight (weights NULL: all 1) and imaging weight that
+ * fall inside the uv window floor(x) > -W/2, ceil(x) < W/2 (likewise y), and
+ * the longest such baseline in lambda; 0/0 when no sample qualifies. */
+void orc_calculate_w_limits(const orc_ms_settings* s, size_t nrow, size_t nchan,
+                            const double* uvw_m, const double* wavelengths,
+                            const float* weights, void* iw, double* min_w,
+                            double* max_w, double* max_baseline_uvw);
+/* Centre-keeping trim (w x h -> tw x th) and untrim with zero fill. */
+void orc_trim_image(const float* in, size_t w, size_t h, size_t tw, size_t th,
+                    float* out);
+void orc_untrim_image(const float* in, size_t tw, size_t th, size_t w, size_t h,
+                      float* out);
+
 #ifdef __cplusplus
 }
 #endif

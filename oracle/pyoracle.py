@@ -113,6 +113,10 @@ def _setup(L):
         "orc_dft_invert": (None, [sz, sz, d, d, d, d, sz, pd, pf, d, pd, sz]),
         "orc_dft_predict": (None, [sz, sz, d, d, d, d, sz, pd, pd, pd, sz]),
         "orc_fft2d": (None, [sz, sz, i, pd]),
+        "orc_calculate_w_limits": (None, [C.POINTER(MsSettings), sz, sz, pd, pd, pf, vp, pd,
+                                          pd, pd]),
+        "orc_trim_image": (None, [pf, sz, sz, sz, sz, pf]),
+        "orc_untrim_image": (None, [pf, sz, sz, sz, sz, pf]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -358,3 +362,32 @@ def fft2d(data, sign):
     h, w = a.shape
     lib().orc_fft2d(w, h, sign, a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)))
     return a
+
+
+def calculate_w_limits(settings, uvw_m, wavelengths, image_weights, weights=None):
+    """MSGridderBase::calculateWLimits restatement: (min_w, max_w,
+    max_baseline_uvw) in wavelengths."""
+    uvw = _c(uvw_m, np.float64).reshape(-1, 3)
+    wl = _c(wavelengths, np.float64)
+    w = _c(weights, np.float32) if weights is not None else None
+    lo, hi, bl = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_calculate_w_limits(C.byref(settings), uvw.shape[0], wl.shape[0],
+                                 _p(uvw, C.c_double), _p(wl, C.c_double), _p(w, C.c_float),
+                                 image_weights.h, C.byref(lo), C.byref(hi), C.byref(bl))
+    return lo.value, hi.value, bl.value
+
+
+def trim_image(image, trim_width, trim_height):
+    img = _c(image, np.float32)
+    h, w = img.shape
+    out = np.zeros((trim_height, trim_width), np.float32)
+    lib().orc_trim_image(_p(img, C.c_float), w, h, trim_width, trim_height, _p(out, C.c_float))
+    return out
+
+
+def untrim_image(image, width, height):
+    img = _c(image, np.float32)
+    th, tw = img.shape
+    out = np.zeros((height, width), np.float32)
+    lib().orc_untrim_image(_p(img, C.c_float), tw, th, width, height, _p(out, C.c_float))
+    return out

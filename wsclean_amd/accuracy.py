@@ -54,6 +54,35 @@ def precision_workload(n_samples=100_000, seed=303, n_sources=20, noise=0.1,
                 width=width, cutout=cutout, n_w=32, flux=flux, pixels=px)
 
 
+def direct_reference(wk, device="cuda:0"):
+    """The FP64 direct transform of a precision workload: the dirty image on
+    the central cutout (normalised by 1/n) and the predicted visibilities of
+    its model, as numpy arrays."""
+    import torch
+    from .direct import direct_invert, direct_predict
+    n = wk["uvw_lambda"].shape[0]
+    dev = torch.device(device)
+    uvw = torch.from_numpy(np.ascontiguousarray(wk["uvw_lambda"])).to(dev)
+    one = torch.ones(1, dtype=torch.float64, device=dev)  # uvw already in wavelengths
+    vis = torch.from_numpy(wk["vis"]).to(dev).reshape(n, 1)
+    model = torch.from_numpy(wk["model"]).to(dev)
+    cut, dl = wk["cutout"], wk["dl"]
+    img = direct_invert(uvw, one, vis, cut, cut, dl, dl, multiplication_factor=1.0 / n)
+    v = direct_predict(model, uvw, one, dl, dl)
+    torch.cuda.synchronize()
+    return img.cpu().numpy(), v.cpu().numpy().reshape(-1)
+
+
+def errors(image, vis, dft_img, dft_vis, cutout):
+    """(image RMS, vis RMS) relative to the direct peaks, image on the
+    central cutout of a full-size image, as report() computes them."""
+    lo = (image.shape[-1] - cutout) // 2
+    im = np.asarray(image, np.float64)[lo:lo + cutout, lo:lo + cutout]
+    pv = np.asarray(vis, np.complex128).reshape(-1)
+    ok = np.isfinite(pv)
+    return _rel(im, dft_img)[0], _rel(pv[ok], dft_vis[ok])[0]
+
+
 def _rel(a, b):
     peak = float(np.max(np.abs(b)))
     d = np.abs(a - b)
@@ -66,7 +95,6 @@ def report(device="cuda:0", precisions=("mixed", "reference", "plain"), n_sample
     kernel oversampling (63: the benchmark setting; 1023: the reference's
     default, which lowers the kernel's own approximation error)."""
     import torch
-    from .direct import direct_invert, direct_predict
     from .gridder import DeviceGridder
 
     wk = precision_workload(n_samples=n_samples, seed=seed)
@@ -80,10 +108,7 @@ def report(device="cuda:0", precisions=("mixed", "reference", "plain"), n_sample
     aw = np.abs(wk["uvw_lambda"][:, 2])
     w_lo, w_hi = float(aw.min()), float(aw.max())
 
-    dft_img = direct_invert(uvw, one, vis.reshape(n, 1), cut, cut, dl, dl,
-                            multiplication_factor=1.0 / n).cpu().numpy()
-    dft_vis = direct_predict(model, uvw, one, dl, dl).cpu().numpy().reshape(-1)
-    torch.cuda.synchronize()
+    dft_img, dft_vis = direct_reference(wk, device)
 
     out = {"protocol": (f"{n} random (row, channel) samples of the 2016-baseline 3 km-disc "
                         "observation, 20 point sources + 0.1 Jy noise, natural weights; "

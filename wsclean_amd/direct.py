@@ -10,12 +10,19 @@ from __future__ import annotations
 
 import ctypes as C
 
-from . import _lib
+from . import _checks, _lib
 
 
 def _stream(t):
     import torch
     return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _device_of(uvw_m):
+    import torch
+    if not isinstance(uvw_m, torch.Tensor) or uvw_m.device.type != "cuda":
+        raise ValueError("uvw_m must be a tensor on a HIP device (expected cuda)")
+    return _checks._dev_index(uvw_m)
 
 
 def _vis_view(vis, image):
@@ -29,17 +36,21 @@ def direct_invert(uvw_m, wavelengths, vis, width, height, dl, dm, l_shift=0.0, m
     """image[y, x] = mult * sum_k Re(V_k exp(-2 pi i (u l + v m + w n'))), FP64."""
     import torch
     L = _lib.load()
+    dev = _device_of(uvw_m)
+    nrow, nchan = _checks.plan_args(uvw_m, wavelengths, dev)
     vis, stride, offset = _vis_view(vis, image)
-    assert vis.dtype == torch.complex64 and vis.is_contiguous()
-    assert uvw_m.dtype == torch.float64 and wavelengths.dtype == torch.float64
-    nrow, nchan = uvw_m.shape[0], wavelengths.shape[0]
-    assert vis.shape[0] == nrow and vis.shape[1] == nchan
+    if vis.dim() not in (2, 3) or tuple(vis.shape[:2]) != (nrow, nchan):
+        raise ValueError(f"vis must be [nrow, nchan] or [nrow, nchan, n_images] with "
+                         f"nrow={nrow}, nchan={nchan}; got {tuple(vis.shape)}")
+    if not 0 <= offset < stride:
+        raise ValueError(f"image {image} out of range for {stride} images")
+    _checks.tensor(vis, "vis", torch.complex64, device=dev)
     if out is None:
         out = torch.empty((height, width), dtype=torch.float64, device=vis.device)
-    assert out.dtype == torch.float64 and out.is_contiguous() and out.numel() == width * height
+    _checks.tensor(out, "out", torch.float64, numel=width * height, device=dev)
     vptr = vis.data_ptr() + offset * 8
     _lib.check(L.hws_direct_invert_device(
-        vis.device.index or 0, _stream(vis), width, height, dl, dm, l_shift, m_shift, nrow,
+        dev, _stream(vis), width, height, dl, dm, l_shift, m_shift, nrow,
         nchan, C.c_void_p(uvw_m.data_ptr()), C.c_void_p(wavelengths.data_ptr()),
         C.c_void_p(vptr), stride, multiplication_factor, C.c_void_p(out.data_ptr())))
     return out
@@ -50,15 +61,23 @@ def direct_predict(model, uvw_m, wavelengths, dl, dm, l_shift=0.0, m_shift=0.0, 
     """V_k = sum_(x,y) I(x, y) exp(+2 pi i (u l + v m + w n')), FP64 sums."""
     import torch
     L = _lib.load()
-    assert model.dtype == torch.float32 and model.is_contiguous() and model.dim() == 2
+    dev = _device_of(uvw_m)
+    nrow, nchan = _checks.plan_args(uvw_m, wavelengths, dev)
+    if model is None or model.dim() != 2:
+        raise ValueError("model must be a float32 [H, W] tensor")
+    _checks.tensor(model, "model", torch.float32, device=dev)
     height, width = model.shape
-    nrow, nchan = uvw_m.shape[0], wavelengths.shape[0]
     if out is None:
         out = torch.empty((nrow, nchan), dtype=torch.complex64, device=model.device)
     out_, stride, offset = _vis_view(out, image)
-    assert out_.is_contiguous() and out_.dtype == torch.complex64
+    if out_.dim() not in (2, 3) or tuple(out_.shape[:2]) != (nrow, nchan):
+        raise ValueError(f"out must be [nrow, nchan] or [nrow, nchan, n_images] with "
+                         f"nrow={nrow}, nchan={nchan}; got {tuple(out_.shape)}")
+    if not 0 <= offset < stride:
+        raise ValueError(f"image {image} out of range for {stride} images")
+    _checks.tensor(out_, "out", torch.complex64, device=dev)
     _lib.check(L.hws_direct_predict_device(
-        model.device.index or 0, _stream(model), width, height, dl, dm, l_shift, m_shift,
+        dev, _stream(model), width, height, dl, dm, l_shift, m_shift,
         C.c_void_p(model.data_ptr()), nrow, nchan, C.c_void_p(uvw_m.data_ptr()),
         C.c_void_p(wavelengths.data_ptr()), C.c_void_p(out_.data_ptr() + offset * 8), stride))
     return out

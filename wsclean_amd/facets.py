@@ -194,13 +194,21 @@ class FacetBatch:
 
     def invert(self, vis, multiplication_factor, reals, imags=None):
         """Eager invert of every facet, one after the other."""
+        self._check_count(reals, imags)
         for i, g in enumerate(self.gridders):
             g.invert(vis, multiplication_factor, reals[i], imags[i] if imags else None)
 
     def predict(self, reals, imags, vis_outs):
         """Eager predict of every facet into its own visibility tensor."""
+        self._check_count(reals, imags, vis_outs)
         for i, g in enumerate(self.gridders):
             g.predict(reals[i], imags[i] if imags else None, vis_outs[i])
+
+    def _check_count(self, *lists):
+        n = len(self.gridders)
+        for lst in lists:
+            if lst is not None and len(lst) != n:
+                raise ValueError(f"expected one tensor per facet ({n}), got {len(lst)}")
 
     def _arrays(self, tensors):
         arr = (C.c_void_p * len(self.gridders))()
@@ -213,6 +221,9 @@ class FacetBatch:
         """One HIP graph that inverts every facet (one branch per facet).
         Valid while the plans and the tensors stay alive."""
         n = len(self.gridders)
+        self._check_count(reals, imags)
+        for i, g in enumerate(self.gridders):
+            g.check_invert_args(vis, reals[i], imags[i] if imags else None)
         hs = (C.c_void_p * n)(*[g.h.value for g in self.gridders])
         vs = (C.c_void_p * n)(*([vis.data_ptr()] * n))
         mult = (C.c_double * n)(*([float(multiplication_factor)] * n))
@@ -226,6 +237,9 @@ class FacetBatch:
     def capture_predict(self, reals, imags, vis_outs):
         """One HIP graph that predicts every facet into its own output."""
         n = len(self.gridders)
+        self._check_count(reals, imags, vis_outs)
+        for i, g in enumerate(self.gridders):
+            g.check_predict_args(reals[i], imags[i] if imags else None, vis_outs[i])
         hs = (C.c_void_p * n)(*[g.h.value for g in self.gridders])
         gr = C.c_void_p()
         g0 = self.gridders[0]

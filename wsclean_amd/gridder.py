@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _lib
+from . import _checks, _lib
 from ._lib import HwsConfig, HwsStats, KERNEL_MODES, PRECISIONS, check
 
 SPEED_OF_LIGHT = 299792458.0
@@ -215,16 +215,31 @@ class DeviceGridder(_Context):
         return self.n_passes
 
     def plan(self, uvw_m, wavelengths):
+        nrow, nchan = _checks.plan_args(uvw_m, wavelengths, self.device)
         self._uvw, self._wl = uvw_m, wavelengths  # keep alive
-        self.nrow, self.nchan = uvw_m.shape[0], wavelengths.shape[0]
+        self.nrow, self.nchan = nrow, nchan
         self._check(self.L.hws_plan_device(self.h, self.nrow, self.nchan, _ptr(uvw_m),
                                            _ptr(wavelengths)))
 
+    def check_invert_args(self, vis, real_out, imag_out):
+        """Raises ValueError unless vis is complex64 with nrow * nchan *
+        n_images elements and the images are float32 [n_images, H, W] (imag
+        required iff an image is complex), all contiguous on this device."""
+        _checks.transfer_args(vis, real_out, imag_out, self.nrow, self.nchan, self.n_images,
+                              self.height, self.width, bool(self.complex_images), self.device)
+
+    def check_predict_args(self, real, imag, vis_out):
+        _checks.transfer_args(vis_out, real, imag, self.nrow, self.nchan, self.n_images,
+                              self.height, self.width, bool(self.complex_images), self.device,
+                              vis_name="vis_out", what="model")
+
     def invert(self, vis, multiplication_factor, real_out, imag_out=None):
+        self.check_invert_args(vis, real_out, imag_out)
         self._check(self.L.hws_invert_planned_device(self.h, _ptr(vis), multiplication_factor,
                                                      _ptr(real_out), _ptr(imag_out)))
 
     def predict(self, real, imag, vis_out):
+        self.check_predict_args(real, imag, vis_out)
         self._check(self.L.hws_predict_planned_device(self.h, _ptr(real), _ptr(imag),
                                                       _ptr(vis_out)))
 
@@ -234,6 +249,7 @@ class DeviceGridder(_Context):
     def capture_invert(self, vis, multiplication_factor, real_out, imag_out=None):
         """A HIP graph of invert() on the planned samples (single pass);
         replay with .launch(). Valid until the next plan()."""
+        self.check_invert_args(vis, real_out, imag_out)
         gr = C.c_void_p()
         self._check(self.L.hws_graph_capture_invert(self.h, _ptr(vis), multiplication_factor,
                                                     _ptr(real_out), _ptr(imag_out),
@@ -242,6 +258,7 @@ class DeviceGridder(_Context):
 
     def capture_predict(self, real, imag, vis_out):
         """A HIP graph of predict() on the planned samples (single pass)."""
+        self.check_predict_args(real, imag, vis_out)
         gr = C.c_void_p()
         self._check(self.L.hws_graph_capture_predict(self.h, _ptr(real), _ptr(imag),
                                                      _ptr(vis_out), C.byref(gr)))

@@ -125,3 +125,22 @@ def source_visibilities(uvw_lambda, l, m, flux):
 def pixel_of(l, m, width, height, dl, dm):
     """Image pixel (x, y) of (l, m): l = (W/2 - x) dl, m = (y - H/2) dm."""
     return width // 2 - l / dl, height // 2 + m / dm
+
+
+def config4_facets(seed=404, n=16, lo=512, hi=2048, radius_deg=5.0):
+    """The batching workload's facet set: n facets whose boxes are uniform in
+    [lo, hi] pixels, each gridded at the next multiple of 64 with no prime
+    factor above 7 (the low-prime rule of math/calculatefftsize.h:4-19), with
+    centres uniform over the disc of radius_deg about the phase centre; the
+    facet's l/m shift is the direction cosine of its centre."""
+    from .facets import Facet, gridded_size
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        box = int(rng.integers(lo, hi + 1))
+        size = gridded_size(box, 1.0, 64)
+        r = np.deg2rad(radius_deg) * np.sqrt(rng.uniform())
+        th = rng.uniform(0.0, 2.0 * np.pi)
+        out.append(Facet(size, size, float(np.sin(r) * np.cos(th)),
+                         float(np.sin(r) * np.sin(th)), (0, 0, box, box), f"f{i}"))
+    return out
