@@ -86,3 +86,7 @@ def test_direct_transform_rejects_bad_arguments_before_device_use():
     st = L.hws_direct_invert_device(0, None, 33, 16, 0.01, 0.01, 0.0, 0.0, 0, 1, None, None,
                                     None, 1, 1.0, None)
     assert st == 1 and b"even" in L.hws_last_error(None)
+    fake = C.c_void_p(1 << 20)  # never dereferenced: rejected by the size check
+    st = L.hws_direct_predict_device(0, None, 16, 16, 0.01, 0.01, 0.0, 0.0, fake, 1 << 28, 16,
+                                     fake, fake, fake, 1)
+    assert st == 1 and b"2^31" in L.hws_last_error(None)

@@ -9,7 +9,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def _make(directory: str, force: bool, *extra: str) -> None:
-    args = ["make", "-s", "-C", directory, *extra]
+    args = ["make", "-s", "-j8", "-C", directory, *extra]
     if force:
         args.insert(1, "-B")
     subprocess.run(args, check=True)

@@ -204,8 +204,11 @@ hws_status validateGeometry(size_t width, size_t height, double dl, double dm, s
     return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "pixel sizes must be positive");
   if (nchan == 0 || nchan > (size_t(1) << 20) || stride == 0)
     return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "bad channel count or stride");
-  if (nrow > (size_t(1) << 40) / nchan)
-    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "too many samples");
+  // one thread per sample in one launch: the grid's work-item count must fit
+  // in 32 bits
+  if (nrow > (size_t(1) << 31) / nchan)
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT,
+                        "too many samples for one launch (at most 2^31)");
   return HWS_OK;
 }
 
