@@ -310,6 +310,25 @@ hws_status hws_direct_predict_device(int device, void* hip_stream, size_t width,
                                      const double* d_wavelengths,
                                      float* d_vis_out, size_t vis_stride);
 
+/* hws_direct_predict_device without allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc): the same
+ * kernels and results as hws_direct_predict_device, but their scratch (the
+ * non-zero pixel list and block counts) is a fixed array the library keeps on
+ * each device until the process ends, so images are limited to 2^20 pixels
+ * (32 MiB of scratch per device; larger images are rejected with
+ * HWS_ERR_INVALID_ARGUMENT). Nothing is allocated or freed and the call
+ * returns once the transform is enqueued on hip_stream. Calls on one device
+ * share that array: calls on one stream are ordered, calls that overlap in
+ * time on one device (other streams or threads) race. */
+hws_status hws_direct_predict_device_noalloc(int device, void* hip_stream, size_t width,
+                                             size_t height, double dl, double dm,
+                                             double l_shift, double m_shift,
+                                             const float* d_image, size_t nrow, size_t nchan,
+                                             const double* d_uvw_m,
+                                             const double* d_wavelengths, float* d_vis_out,
+                                             size_t vis_stride);
+
 #ifdef __cplusplus
 }
 #endif

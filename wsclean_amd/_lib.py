@@ -102,6 +102,8 @@ SIGNATURES = {
                                       _vp, _sz, _d, _vp]),
     "hws_direct_predict_device": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _vp, _sz, _sz, _vp,
                                        _vp, _vp, _sz]),
+    "hws_direct_predict_device_noalloc": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _vp, _sz, _sz,
+                                               _vp, _vp, _vp, _sz]),
 }
 
 

@@ -17,6 +17,7 @@
 #include "hws_internal.h"
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 namespace hws {
@@ -212,6 +213,31 @@ hws_status validateGeometry(size_t width, size_t height, double dl, double dm, s
   return HWS_OK;
 }
 
+// The direct predict's launches on `stream`: non-zero pixels of the model
+// compacted into `terms` (room for every pixel) through the block counts in
+// `counts` (pixBlocks + 1 entries, the last receives the total), then one
+// thread per sample. Every entry of both buffers that a later kernel reads is
+// written first, so neither needs clearing.
+hws_status launchDirectPredict(hipStream_t stream, size_t width, size_t height, double dl,
+                               double dm, double l_shift, double m_shift, const float* d_image,
+                               size_t nrow, size_t nchan, const double* d_uvw_m,
+                               const double* d_wavelengths, float* d_vis_out, size_t vis_stride,
+                               uint32_t* counts, PixelTerm* terms) {
+  const size_t npix = width * height;
+  const size_t pixBlocks = (npix + kDirectThreads - 1) / kDirectThreads;
+  countNonzeroKernel<<<unsigned(pixBlocks), kDirectThreads, 0, stream>>>(d_image, npix, counts);
+  scanCountsKernel<<<1, 1024, 0, stream>>>(counts, pixBlocks);
+  compactNonzeroKernel<<<unsigned(pixBlocks), kDirectThreads, 0, stream>>>(
+      d_image, int(width), int(height), dl, dm, l_shift, m_shift, counts, terms);
+  const size_t ns = nrow * nchan;
+  if (ns)
+    directPredictKernel<<<unsigned((ns + kDirectThreads - 1) / kDirectThreads), kDirectThreads,
+                          0, stream>>>(d_uvw_m, d_wavelengths, nrow, int(nchan), terms,
+                                       counts + pixBlocks, reinterpret_cast<float2*>(d_vis_out),
+                                       vis_stride);
+  return checkHip(nullptr, hipGetLastError(), "direct predict launch");
+}
+
 }  // namespace
 }  // namespace hws
 
@@ -263,18 +289,9 @@ extern "C" hws_status hws_direct_predict_device(int device, void* hip_stream, si
   if (st == HWS_OK)
     st = checkHip(nullptr, hipMalloc(&terms, npix * sizeof(PixelTerm)), "direct predict terms");
   if (st == HWS_OK) {
-    countNonzeroKernel<<<unsigned(pixBlocks), kDirectThreads, 0, stream>>>(d_image, npix,
-                                                                            counts);
-    scanCountsKernel<<<1, 1024, 0, stream>>>(counts, pixBlocks);
-    compactNonzeroKernel<<<unsigned(pixBlocks), kDirectThreads, 0, stream>>>(
-        d_image, int(width), int(height), dl, dm, l_shift, m_shift, counts, terms);
-    const size_t ns = nrow * nchan;
-    if (ns)
-      directPredictKernel<<<unsigned((ns + kDirectThreads - 1) / kDirectThreads),
-                            kDirectThreads, 0, stream>>>(
-          d_uvw_m, d_wavelengths, nrow, int(nchan), terms, counts + pixBlocks,
-          reinterpret_cast<float2*>(d_vis_out), vis_stride);
-    st = checkHip(nullptr, hipGetLastError(), "direct predict launch");
+    st = launchDirectPredict(stream, width, height, dl, dm, l_shift, m_shift, d_image, nrow,
+                             nchan, d_uvw_m, d_wavelengths, d_vis_out, vis_stride, counts,
+                             terms);
     // the scratch buffers are released once the transform has run
     const hws_status sy = checkHip(nullptr, hipStreamSynchronize(stream), "direct predict");
     if (st == HWS_OK) st = sy;
@@ -282,4 +299,80 @@ extern "C" hws_status hws_direct_predict_device(int device, void* hip_stream, si
   if (terms) (void)hipFree(terms);
   if (counts) (void)hipFree(counts);
   return st;
+}
+
+// ---- the direct predict without allocation -------------------------------
+//
+// The scratch of the direct predict (pixel terms and block counts) lives in
+// one file-scope device array instead of hipMalloc'd buffers, for callers
+// that cannot allocate (graph capture, serving loops). Its size is fixed:
+// images of at most kNoAllocMaxPixels pixels. Each device holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device (the array's name in host code
+// is a host-side stand-in, not a device address).
+
+namespace hws {
+namespace {
+
+constexpr size_t kNoAllocMaxPixels = size_t(1) << 20;
+constexpr size_t kNoAllocMaxBlocks = kNoAllocMaxPixels / kDirectThreads;
+constexpr size_t kScratchTermBytes = kNoAllocMaxPixels * sizeof(PixelTerm);
+constexpr size_t kScratchBytes =
+    kScratchTermBytes + (kNoAllocMaxBlocks + 1) * sizeof(uint32_t);  // 32 MiB + 16 KiB
+constexpr int kScratchDevices = 64;
+
+std::mutex scratchMutex;
+void* scratchAddress[kScratchDevices] = {};
+
+}  // namespace
+}  // namespace hws
+
+static __device__ __align__(256) unsigned char scratch[hws::kScratchBytes];
+
+namespace hws {
+namespace {
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the caller has made `device` current).
+hws_status scratchFor(int device, unsigned char** out) {
+  if (device < 0 || device >= kScratchDevices)
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratchMutex);
+  if (!scratchAddress[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return checkHip(nullptr, e, "scratch symbol address");
+    if (!p) return setLastError(nullptr, HWS_ERR_HIP, "scratch symbol address is null");
+    scratchAddress[device] = p;
+  }
+  *out = static_cast<unsigned char*>(scratchAddress[device]);
+  return HWS_OK;
+}
+
+}  // namespace
+}  // namespace hws
+
+extern "C" hws_status hws_direct_predict_device_noalloc(
+    int device, void* hip_stream, size_t width, size_t height, double dl, double dm,
+    double l_shift, double m_shift, const float* d_image, size_t nrow, size_t nchan,
+    const double* d_uvw_m, const double* d_wavelengths, float* d_vis_out, size_t vis_stride) {
+  const hws_status valid = validateGeometry(width, height, dl, dm, nrow, nchan, vis_stride);
+  if (valid != HWS_OK) return valid;
+  if (width * height > kNoAllocMaxPixels)
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT,
+                        "image too large for the no-allocation direct predict: " +
+                            std::to_string(width * height) + " pixels, at most " +
+                            std::to_string(kNoAllocMaxPixels));
+  if (!d_image || (nrow && (!d_uvw_m || !d_wavelengths || !d_vis_out)))
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "null data pointer");
+  DeviceGuard guard(device);
+  if (!guard.ok) return setLastError(nullptr, HWS_ERR_HIP, "hipSetDevice failed");
+  unsigned char* base = nullptr;
+  const hws_status st = scratchFor(device, &base);
+  if (st != HWS_OK) return st;
+  PixelTerm* terms = reinterpret_cast<PixelTerm*>(base);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(base + kScratchTermBytes);
+  return launchDirectPredict(static_cast<hipStream_t>(hip_stream), width, height, dl, dm,
+                             l_shift, m_shift, d_image, nrow, nchan, d_uvw_m, d_wavelengths,
+                             d_vis_out, vis_stride, counts, terms);
 }
