@@ -21,6 +21,7 @@ def _gridder(nrow=10, nchan=3, n_images=2, complex_images=(1,), H=8, W=16):
     g.L, g.h = _NoCalls(), None
     g.nrow, g.nchan, g.n_images = nrow, nchan, n_images
     g.height, g.width, g.complex_images, g.device = H, W, complex_images, 0
+    g.planned = nrow > 0
     return g
 
 
@@ -35,8 +36,7 @@ def test_plan_rejects_bad_uvw_and_wavelengths():
     wl = torch.ones(3, dtype=torch.float64)
     for bad in (torch.zeros((10, 3), dtype=torch.float32), torch.zeros((10, 2), dtype=torch.float64),
                 torch.zeros(30, dtype=torch.float64),
-                torch.zeros((3, 10), dtype=torch.float64).t(),
-                torch.zeros((0, 3), dtype=torch.float64)):
+                torch.zeros((3, 10), dtype=torch.float64).t()):
         with pytest.raises(ValueError):
             g.plan(bad, wl)
     uvw = torch.zeros((10, 3), dtype=torch.float64)
@@ -47,6 +47,9 @@ def test_plan_rejects_bad_uvw_and_wavelengths():
     # right dtypes and shapes, but host tensors: rejected by the device check
     with pytest.raises(ValueError, match="expected cuda:0"):
         g.plan(uvw, wl)
+    # empty sample sets are valid (the library plans nothing)
+    with pytest.raises(ValueError, match="expected cuda:0"):
+        g.plan(torch.zeros((0, 3), dtype=torch.float64), wl)
 
 
 @pytest.mark.parametrize("which", ["vis_dtype", "vis_size", "vis_strided", "real_dtype",

@@ -27,7 +27,13 @@ run-dependent order, so there every image is compared with the scaled oracle
 instead.
 
 Tolerances (relative L2 against the oracle float restatement): mixed 3e-5,
-plain 3e-3, as for the small cases (tests/test_gpu_parity.py).
+as for the small cases (tests/test_gpu_parity.py). The plain form computes
+sample positions, layer indices and phases in FP32: at 4096^2 a position
+u dl W reaches 2048 pixels, where an FP32 rounding of 1.2e-4 pixel is 0.8%
+of the kernel table's 1/63-pixel step, so a share of the samples takes a
+neighbouring kernel offset (or layer) that the FP64 oracle does not. That
+costs about 6e-3 here (3e-4 at 64^2), hence 1e-2 for the plain form at these
+sizes. The batched precisions keep those quantities in FP64 for this reason.
 """
 import os
 
@@ -38,7 +44,7 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"mixed": 3e-5, "reference": 1e-5, "plain": 3e-3}
+TOL = {"mixed": 3e-5, "reference": 1e-5, "plain": 1e-2}
 IMAGES = ("XX", "XY", "YX", "YY")
 COMPLEX = (1, 2)
 ARCSEC = np.pi / (180.0 * 3600.0)
@@ -221,7 +227,8 @@ def test_config2_4096_device_path_matches_oracle(oracle, case4096, precision):
     _say(f"4096^2 x {c.n_w} layers, {c.nrow} rows x {c.nchan} ch x 4, {precision}")
     re, im, out, st = run_device(c, precision)
     assert st["samples_in"] == c.nrow * c.nchan
-    assert st["samples_gridded"] > 0.9 * st["samples_in"]
+    if precision != "plain":  # the plain form keeps no placement counters
+        assert st["samples_gridded"] > 0.9 * st["samples_in"]
     assert np.isfinite(re).all() and np.isfinite(im).all()
     if precision != "plain":
         check_exact_multiples(re, im, out, "4096")
@@ -231,12 +238,24 @@ def test_config2_4096_device_path_matches_oracle(oracle, case4096, precision):
 def test_config2_4096_gridded_layers_match_oracle(oracle, case4096):
     """Layers of the host protocol (the same bin + matrix-core grid kernels at
     65,536 tiles per layer) against the oracle's layers, XX and YX: the three
-    most populated layers of each kind and a sparse one."""
+    most populated layers of each kind and a sparse one. A dense uv cell here
+    sums thousands of FP32 products, so the tolerance is set from the FP32
+    oracle's own distance to its FP64 form on the same layer (the noise floor
+    of FP32 gridding at this size): three times that, and never below the
+    small cases' 2e-6."""
     import wsclean_amd
     c = case4096
     for k in (0, 2):
         cplx = k in COMPLEX
         _, lay_o, _ = c.oracle(oracle, k)
+        g = oracle.RefGridder(c.width, c.width, c.pixel, c.pixel, ORACLE_THREADS, 7, 63,
+                              "double")
+        g.set_is_complex(cplx)
+        g.prepare_w_layers(c.n_w, 1e13, c.min_w, c.max_w)
+        g.start_inversion_pass(0)
+        g.add_samples(c.uvw_l, c.vis[..., k].reshape(-1))
+        lay_d = {i: g.layer(i) for i in lay_o}
+        del g
         h = wsclean_amd.WStackingGridder(c.width, c.width, c.pixel, c.pixel, 7, 63,
                                          precision="mixed", n_images=1,
                                          complex_images=(0,) if cplx else ())
@@ -244,9 +263,12 @@ def test_config2_4096_gridded_layers_match_oracle(oracle, case4096):
         h.start_inversion_pass(0)
         h.add_data(c.uvw, c.wl, c.vis[..., k:k + 1])
         for i, ref in lay_o.items():
+            floor = _rel(ref, lay_d[i])
             e = _rel(h.get_gridded_uv_layer(i), ref)
-            _say(f"4096 {IMAGES[k]} layer {i}: rel {e:.2e}")
-            assert e < 2e-6, (IMAGES[k], i, e)
+            e_d = _rel(h.get_gridded_uv_layer(i), lay_d[i])
+            _say(f"4096 {IMAGES[k]} layer {i}: rel {e:.2e} vs oracle FP32 ({e_d:.2e} vs FP64; "
+                 f"oracle FP32 vs FP64 {floor:.2e})")
+            assert e < max(2e-6, 3.0 * floor), (IMAGES[k], i, e, floor)
         h.close()
 
 

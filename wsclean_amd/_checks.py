@@ -50,8 +50,6 @@ def plan_args(uvw_m, wavelengths, device):
     if wavelengths is None or getattr(wavelengths, "dim", lambda: 0)() != 1:
         raise ValueError("wavelengths must be a float64 tensor of shape [nchan]")
     tensor(wavelengths, "wavelengths", torch.float64)
-    if uvw_m.shape[0] == 0 or wavelengths.shape[0] == 0:
-        raise ValueError("no samples: uvw_m and wavelengths must be non-empty")
     on_device([("uvw_m", uvw_m), ("wavelengths", wavelengths)], device)
     return uvw_m.shape[0], wavelengths.shape[0]
 
@@ -76,18 +74,16 @@ def image_args(real, imag, n_images, height, width, has_complex, device, what="i
 def vis_args(vis, nrow, nchan, n_images, device, name="vis"):
     """complex64 [nrow, nchan, n_images] (any shape with that element count)."""
     import torch
-    if nrow == 0 or nchan == 0:
-        raise ValueError("no plan: call plan() with the samples first")
     tensor(vis, name, torch.complex64, numel=nrow * nchan * n_images)
     on_device([(name, vis)], device)
 
 
 def transfer_args(vis, real, imag, nrow, nchan, n_images, height, width, has_complex, device,
-                  vis_name="vis", what="image"):
+                  vis_name="vis", what="image", planned=True):
     """The tensors of one invert or predict: structure of all of them first
     (vis_args, image_args without the device), then their device."""
     import torch
-    if nrow == 0 or nchan == 0:
+    if not planned:
         raise ValueError("no plan: call plan() with the samples first")
     tensor(vis, vis_name, torch.complex64, numel=nrow * nchan * n_images)
     shp = (n_images, height, width)

@@ -197,6 +197,7 @@ class DeviceGridder(_Context):
         self.device = device
         self.n_passes = 0
         self.nrow = self.nchan = 0
+        self.planned = False
         # order the gridder's work with torch's on the same stream unless the
         # caller names another one
         if stream is None:
@@ -218,20 +219,23 @@ class DeviceGridder(_Context):
         nrow, nchan = _checks.plan_args(uvw_m, wavelengths, self.device)
         self._uvw, self._wl = uvw_m, wavelengths  # keep alive
         self.nrow, self.nchan = nrow, nchan
+        self.planned = False
         self._check(self.L.hws_plan_device(self.h, self.nrow, self.nchan, _ptr(uvw_m),
                                            _ptr(wavelengths)))
+        self.planned = True
 
     def check_invert_args(self, vis, real_out, imag_out):
         """Raises ValueError unless vis is complex64 with nrow * nchan *
         n_images elements and the images are float32 [n_images, H, W] (imag
         required iff an image is complex), all contiguous on this device."""
         _checks.transfer_args(vis, real_out, imag_out, self.nrow, self.nchan, self.n_images,
-                              self.height, self.width, bool(self.complex_images), self.device)
+                              self.height, self.width, bool(self.complex_images), self.device,
+                              planned=self.planned)
 
     def check_predict_args(self, real, imag, vis_out):
         _checks.transfer_args(vis_out, real, imag, self.nrow, self.nchan, self.n_images,
                               self.height, self.width, bool(self.complex_images), self.device,
-                              vis_name="vis_out", what="model")
+                              vis_name="vis_out", what="model", planned=self.planned)
 
     def invert(self, vis, multiplication_factor, real_out, imag_out=None):
         self.check_invert_args(vis, real_out, imag_out)
