@@ -32,8 +32,9 @@ sample positions, layer indices and phases in FP32: at 4096^2 a position
 u dl W reaches 2048 pixels, where an FP32 rounding of 1.2e-4 pixel is 0.8%
 of the kernel table's 1/63-pixel step, so a share of the samples takes a
 neighbouring kernel offset (or layer) that the FP64 oracle does not. That
-costs about 6e-3 here (3e-4 at 64^2), hence 1e-2 for the plain form at these
-sizes. The batched precisions keep those quantities in FP64 for this reason.
+costs about 6e-3 here (2e-7 at 64^2, where positions stay small), hence 1e-2 for
+the plain form at these sizes. The batched precisions keep those quantities in
+FP64 for this reason.
 """
 import os
 
