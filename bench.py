@@ -88,24 +88,53 @@ class Workload:
         # seeded complex Gaussian visibilities (kernel cost does not depend on values)
         self.vis = torch.randn((self.nrow, self.nchan, 4), dtype=torch.complex64, device=device,
                                generator=g)
+        # seeded visibility weights, uniform in [0.5, 1.5) per visibility
+        self.weights = torch.rand((self.nrow, self.nchan, 4), dtype=torch.float32, device=device,
+                                  generator=g) + 0.5
         self.vis_out = torch.empty_like(self.vis)
         self.real = torch.zeros((4, self.H, self.W), dtype=torch.float32, device=device)
         self.imag = torch.zeros_like(self.real)
 
 
-def run_form(wl_, precision, steps, warmup, tile, device, chunk=0, cycle=True):
+def image_weights(wl_, device):
+    """Briggs robust 0 imaging weights of the workload's samples, gridded
+    from the XX visibility weights (ImageWeights::Grid over one correlation)
+    and finished."""
+    from wsclean_amd.weights import DeviceImageWeights
+    iw = DeviceImageWeights("briggs", wl_.W, wl_.H, wl_.PIXEL, wl_.PIXEL, robust=0.0,
+                            device=device.index or 0)
+    iw.grid(wl_.uvw, wl_.wl, wl_.weights_xx)
+    iw.finish()
+    return iw
+
+
+def run_form(wl_, precision, steps, warmup, tile, device, chunk=0, cycle=True, weighted=True):
     import torch
     import wsclean_amd
     g = wsclean_amd.DeviceGridder(wl_.W, wl_.H, wl_.PIXEL, wl_.PIXEL, 7, 63, precision=precision,
                                   n_images=4, complex_images=wl_.COMPLEX, tile_size=tile,
                                   work_chunk=chunk)
     npass = g.prepare_w_layers(wl_.NW, wl_.min_w, wl_.max_w)
-    phases = {"bin": 0.0, "grid": 0.0, "fft_inv": 0.0, "project": 0.0,
+    phases = {"weights": 0.0, "bin": 0.0, "grid": 0.0, "fft_inv": 0.0, "project": 0.0,
               "img2layer": 0.0, "fft_pred": 0.0, "degrid": 0.0}
     stats_inv = stats_pred = None
+    if weighted and not hasattr(wl_, "weights_xx"):
+        wl_.weights_xx = wl_.weights[..., :1].contiguous()
+    iw_keep = []
 
     def step(record):
         nonlocal stats_inv, stats_pred
+        if weighted:
+            # a step builds the imaging weights, plans (the imaging weight of
+            # every sample and the per-image weight sums), inverts with each
+            # image normalised by its weight sum, and predicts
+            t = time.perf_counter()
+            iw = image_weights(wl_, device)
+            g.set_weighting(iw, wl_.weights, "normal")
+            iw_keep[:] = [iw]  # the previous step's weights are released after re-attaching
+            if record:
+                torch.cuda.synchronize()
+                phases["weights"] += (time.perf_counter() - t) * 1e3
         g.plan(wl_.uvw, wl_.wl)
         g.invert(wl_.vis, 1.0, wl_.real, wl_.imag)
         if record:

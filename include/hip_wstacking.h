@@ -133,6 +133,13 @@ typedef struct hws_stats {
    * reads: the columns the forward column pass transforms. */
   uint64_t tile_cols_total;
   uint64_t tile_cols_needed[2];
+  /* Weighted contexts (hws_set_weighting), per image, over the samples of the
+   * plan whose w-layer the image grids: the counters MSGridderBase keeps
+   * (gridding/msgridderbase.h:197-213, msgridderbase.cpp:640-659). */
+  double total_weight[4];
+  double visibility_weight_sum[4];
+  double max_gridded_weight[4];
+  uint64_t gridded_visibility_count[4];
 } hws_stats;
 
 /* Replaces: WStackingGridder<T>::WStackingGridder (gridding/wstackinggridder.h:120-122,
@@ -239,6 +246,119 @@ hws_status hws_get_stats(const hws_gridder* g, hws_stats* out);
 /* Wait for all work of the context. */
 hws_status hws_synchronize(hws_gridder* g);
 
+/* ---- imaging weights on the device ----
+ * An ImageWeights (structures/imageweights.h:19-129) whose grid lives on one
+ * device, gridded from device-resident rows, and the weighting a context
+ * applies to the visibilities it grids. */
+typedef enum hws_weight_mode {
+  HWS_WEIGHT_NATURAL = 0,
+  HWS_WEIGHT_UNIFORM = 1,
+  HWS_WEIGHT_BRIGGS = 2
+} hws_weight_mode;
+
+typedef struct hws_weights_config {
+  int mode;                 /* hws_weight_mode */
+  double briggs_robustness; /* R of Briggs weighting */
+  size_t width, height;     /* image size; the grid is width x height / 2 */
+  double pixel_scale_x, pixel_scale_y; /* radians */
+  int weights_as_taper;     /* non-zero visibility weights count as 1 */
+  double super_weight;      /* grid cells are super_weight pixels wide */
+  int device;               /* HIP device ordinal */
+} hws_weights_config;
+
+typedef struct hws_weights hws_weights;
+
+/* Replaces: ImageWeights::ImageWeights (structures/imageweights.cpp:33-49):
+ * a zeroed (width/super_weight) x (height/super_weight)/2 grid, sizes rounded
+ * and made even as the reference makes them. */
+hws_status hws_weights_create(const hws_weights_config* config, hws_weights** out);
+void hws_weights_destroy(hws_weights* w);
+/* Replaces: ImageWeights::Grid(MSProvider&, const BandData&)
+ * (structures/imageweights.cpp:75-111) for rows in device memory: every
+ * weight d_weights[row][chan][pol] (NULL: 1) is added to the cell of
+ * (u, v) = uvw / lambda in the v >= 0 half plane and to the total. Sums are
+ * bit-equal to the reference's row-order sums whenever they are exact in FP64
+ * (e.g. unit or few-digit weights), within FP64 rounding otherwise. Enqueued
+ * on hip_stream; may be called once per MeasurementSet. */
+hws_status hws_weights_grid_device(hws_weights* w, void* hip_stream, size_t nrow,
+                                   size_t nchan, size_t npol, const double* d_uvw_m,
+                                   const double* d_wavelengths, const float* d_weights);
+/* Replaces: ImageWeights::FinishGridding (structures/imageweights.cpp:113-144):
+ * natural 1 / uniform 1 / sum / Briggs 1 / (1 + sum s^2). Synchronises
+ * hip_stream; fails with HWS_ERR_BAD_STATE when called twice. */
+hws_status hws_weights_finish(hws_weights* w, void* hip_stream);
+/* Replaces: ImageWeights::Width/Height (structures/imageweights.h:65-66). */
+hws_status hws_weights_shape(const hws_weights* w, size_t* width, size_t* height);
+/* Replaces: ImageWeights::GetGrid (structures/imageweights.h:60): the grid
+ * ([height/2][width] doubles, host memory; NULL to skip) and the total of the
+ * gridded weights (NULL to skip). Synchronises the device. */
+hws_status hws_weights_get_grid(const hws_weights* w, double* grid, double* total_sum);
+
+/* Replaces: the weighting of MSGridderBase::readAndWeightVisibilities
+ * (gridding/msgridderbase.cpp:563-589, 640-659) on the planned device path:
+ * every visibility is gridded times (visibility weight x imaging weight), the
+ * visibility weight d_vis_weights[row][chan][image] (NULL: 1) taken as given
+ * (mode 0), squared (1) or as 1 where non-zero (2), the imaging weight
+ * GetWeight(u, v) of `weights` (finished, same device; NULL detaches). Each
+ * invert then multiplies image i by multiplication_factor /
+ * total_weight[i] (WSMSGridder finalizes with 1 / totalWeight,
+ * gridding/wsmsgridder.cpp:436), and hws_get_stats reports the sums over the
+ * samples whose w-layer the image grids. Applies to hws_plan_device and the
+ * *_planned_device calls and graphs; the host-pointer protocol grids its rows
+ * as given. The weights and d_vis_weights must outlive the context's use of
+ * them. */
+hws_status hws_set_weighting(hws_gridder* g, const hws_weights* weights,
+                             const float* d_vis_weights, int visibility_weighting_mode);
+
+/* ---- staged host rows (a host caller of the planned device path) ----
+ * The MSGridderBase plugin reads and weights its rows on the host and stages
+ * them here batch by batch: each batch is copied into one of two pinned host
+ * buffers and uploaded asynchronously while the caller reads the next one.
+ * hws_stage_end plans the staged samples once; hws_invert_staged and
+ * hws_predict_staged then run every pass on the device, and convert between
+ * the context's (inversion) size and the caller's image and trimmed sizes on
+ * the device (hws_resample_device, hws_trim_device, hws_untrim_device), as
+ * WSMSGridder::Invert and ::Predict do (gridding/wsmsgridder.cpp:446-484,
+ * 510-551). No single reference counterpart: together they replace
+ * WSMSGridder's gridMeasurementSet / predictMeasurementSet row loops
+ * (gridding/wsmsgridder.cpp:130-253, 255-365) with their lanes and passes.
+ *
+ * hws_stage_begin reserves device room for max_rows rows of nchan channels
+ * (wavelengths: host, metres) with (with_visibilities != 0) or without
+ * visibilities; hws_stage_rows appends nrow rows: uvw_m [nrow][3] host, vis
+ * [nrow][nchan][n_images] complex64 host (pre-weighted; NULL without
+ * visibilities). */
+hws_status hws_stage_begin(hws_gridder* g, size_t max_rows, size_t nchan,
+                           const double* wavelengths, int with_visibilities);
+hws_status hws_stage_rows(hws_gridder* g, size_t nrow, const double* uvw_m, const float* vis);
+hws_status hws_stage_end(hws_gridder* g);
+/* Invert of the staged visibilities, times multiplication_factor (and each
+ * image by its own weight sum when a weighting is attached), resampled to
+ * image_width x image_height when that differs from the context size, then
+ * trimmed to trim_width x trim_height: real (imag for complex images, may be
+ * NULL) host [n_images][trim_height][trim_width]. Synchronous. */
+hws_status hws_invert_staged(hws_gridder* g, double multiplication_factor,
+                             size_t image_width, size_t image_height, size_t trim_width,
+                             size_t trim_height, float* real, float* imag);
+/* Predict at the staged rows from host models of trim_width x trim_height
+ * (untrimmed to image_width x image_height, resampled to the context size):
+ * vis_out host [rows][nchan][n_images] complex64, NaN where not predicted.
+ * Synchronous. */
+hws_status hws_predict_staged(hws_gridder* g, const float* real, const float* imag,
+                              size_t trim_width, size_t trim_height, size_t image_width,
+                              size_t image_height, float* vis_out);
+/* The same image ends for the host-pointer protocol: hws_finalize_image
+ * followed by the resampling and trim; hws_initialize_prediction preceded by
+ * the untrim and resampling. */
+hws_status hws_finalize_image_resampled(hws_gridder* g, double multiplication_factor,
+                                        size_t image_width, size_t image_height,
+                                        size_t trim_width, size_t trim_height, float* real,
+                                        float* imag);
+hws_status hws_initialize_prediction_resampled(hws_gridder* g, const float* real,
+                                               const float* imag, size_t trim_width,
+                                               size_t trim_height, size_t image_width,
+                                               size_t image_height);
+
 /* ---- graph capture of the device-resident forms (no reference
  * counterpart) ----
  * hws_graph_capture_invert / _predict capture the whole
@@ -309,6 +429,48 @@ hws_status hws_direct_predict_device(int device, void* hip_stream, size_t width,
                                      size_t nchan, const double* d_uvw_m,
                                      const double* d_wavelengths,
                                      float* d_vis_out, size_t vis_stride);
+
+/* Host-pointer forms of the two transforms above, for host callers (the
+ * DirectFT plugin gridder): arguments as there with vis_stride 1, the image
+ * FP64 [height][width] host memory (invert) or float32 (predict), uvw,
+ * wavelengths and visibilities host memory. They upload, transform and
+ * download on the null stream of `device` and return when done. */
+hws_status hws_direct_invert(int device, size_t width, size_t height, double dl, double dm,
+                             double l_shift, double m_shift, size_t nrow, size_t nchan,
+                             const double* uvw_m, const double* wavelengths, const float* vis,
+                             double multiplication_factor, double* image);
+hws_status hws_direct_predict(int device, size_t width, size_t height, double dl, double dm,
+                              double l_shift, double m_shift, const float* image, size_t nrow,
+                              size_t nchan, const double* uvw_m, const double* wavelengths,
+                              float* vis_out);
+
+/* ---- small inversion and trimming on the device (no context) ----
+ * Images are float32 [n_images][height][width] device arrays; work is ordered
+ * on hip_stream of `device`. Sizes are even, at most 65536.
+ *
+ * Replaces: schaapcommon::fft::Resampler::Resample as WSMSGridder applies it
+ * for small inversion (gridding/wsmsgridder.cpp:452-474 interpolates the
+ * inverted in_w x in_h image up to the image size, :521-551 decimates the
+ * model down to the inversion size): the real-to-complex transform of each
+ * image, the low-frequency block both sizes share copied into the output's
+ * half spectrum, the complex-to-real transform and 1 / (in_w in_h),
+ * evaluated in FP64 as two separable resampling matrices. Allocates scratch
+ * and returns after the resampling has completed. */
+hws_status hws_resample_device(int device, void* hip_stream, const float* d_in, size_t in_w,
+                               size_t in_h, float* d_out, size_t out_w, size_t out_h,
+                               size_t n_images);
+/* Replaces: aocommon::Image::Trim (gridding/wsmsgridder.cpp:476-484): the
+ * trim_width x trim_height centre of each image (offset ((w - tw)/2,
+ * (h - th)/2)). Returns once enqueued. */
+hws_status hws_trim_device(int device, void* hip_stream, const float* d_in, size_t width,
+                           size_t height, float* d_out, size_t trim_width, size_t trim_height,
+                           size_t n_images);
+/* Replaces: aocommon::Image::Untrim (gridding/wsmsgridder.cpp:516-520): each
+ * trimmed image placed at the centre of a zeroed width x height image.
+ * Returns once enqueued. */
+hws_status hws_untrim_device(int device, void* hip_stream, const float* d_in, size_t trim_width,
+                             size_t trim_height, float* d_out, size_t width, size_t height,
+                             size_t n_images);
 
 /* hws_direct_predict_device without allocation: no counterpart in the
  * reference. For callers that cannot allocate device memory (code under HIP

@@ -45,6 +45,7 @@ GridderSettings toSettings(const orc_ms_settings* s) {
   o.memory = s->memory;
   o.wLimit = s->w_limit;
   o.psfMode = s->psf_mode != 0;
+  o.smallInversion = s->small_inversion != 0;
   return o;
 }
 }  // namespace
@@ -382,6 +383,12 @@ void orc_calculate_w_limits(const orc_ms_settings* s, size_t nrow, size_t nchan,
   ms.weights = weights;
   CalculateWLimits(gs, ms, *static_cast<ImageWeightsRef*>(iw), *min_w, *max_w,
                    *max_baseline_uvw);
+}
+
+void orc_resample(const double* in, size_t in_w, size_t in_h, double* out, size_t out_w,
+                  size_t out_h) {
+  std::vector<double> r = Resample(in, in_w, in_h, out_w, out_h);
+  std::memcpy(out, r.data(), r.size() * sizeof(double));
 }
 
 void orc_trim_image(const float* in, size_t w, size_t h, size_t tw, size_t th,

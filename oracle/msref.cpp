@@ -259,6 +259,57 @@ void CalculateWLimits(const GridderSettings& s, const MsView& ms,
   maxWOut = maxW;
 }
 
+std::vector<double> Resample(const double* in, size_t inW, size_t inH, size_t outW,
+                             size_t outH) {
+  // forward transform of the real input (as complex: the half spectrum is
+  // columns 0 .. inW / 2 of it)
+  std::vector<std::complex<double>> x(inW * inH);
+  for (size_t i = 0; i != inW * inH; ++i) x[i] = in[i];
+  Fft2D<double> fwd(inW, inH, -1);
+  fwd.Transform(x.data());
+  const size_t minW = std::min(inW / 2 + 1, outW / 2 + 1);
+  const size_t minH = std::min(inH, outH);
+  // the copied half spectrum: rows 0 .. minH/2 - 1 to the same rows, the last
+  // minH/2 input rows to the last minH/2 output rows
+  std::vector<std::complex<double>> y(outH * minW, 0.0);
+  for (size_t r = 0; r != minH / 2; ++r) {
+    const size_t oldY = inH - minH / 2 + r, newY = outH - minH / 2 + r;
+    for (size_t c = 0; c != minW; ++c) y[newY * minW + c] = x[oldY * inW + c];
+  }
+  for (size_t r = 0; r != minH / 2; ++r)
+    for (size_t c = 0; c != minW; ++c) y[r * minW + c] = x[r * inW + c];
+  // complex-to-real transform of the out_w x out_h Hermitian half spectrum:
+  // inverse transform of each column along y, then along x the real part of
+  // sum_k c_k z_k exp(2 pi i x k / out_w), c_0 = 1, c_k = 2 below the
+  // output's Nyquist column, 1 at it
+  std::vector<std::complex<double>> col(outH);
+  Fft1D<double> colInv(outH, +1);
+  std::vector<std::complex<double>> scratch(outH);
+  for (size_t c = 0; c != minW; ++c) {
+    for (size_t r = 0; r != outH; ++r) col[r] = y[r * minW + c];
+    colInv.Transform(col.data(), scratch.data());
+    for (size_t r = 0; r != outH; ++r) y[r * minW + c] = col[r];
+  }
+  const long double pi = 3.141592653589793238462643383279502884L;
+  std::vector<std::complex<double>> tw(outW);
+  for (size_t k = 0; k != outW; ++k) {
+    const long double a = 2.0L * pi * (long double)k / (long double)outW;
+    tw[k] = std::complex<double>(double(std::cos(a)), double(std::sin(a)));
+  }
+  std::vector<double> out(outW * outH);
+  const double factor = 1.0 / (double(inW) * double(inH));
+  for (size_t r = 0; r != outH; ++r)
+    for (size_t xo = 0; xo != outW; ++xo) {
+      double acc = 0.0;
+      for (size_t c = 0; c != minW; ++c) {
+        const double ck = (c == 0 || 2 * c == outW) ? 1.0 : 2.0;
+        acc += ck * (y[r * minW + c] * tw[(xo * c) % outW]).real();
+      }
+      out[r * outW + xo] = acc * factor;
+    }
+  return out;
+}
+
 size_t SuggestedWGridSize(const GridderSettings& s, double minW, double maxW) {
   const size_t tw = s.trimWidth ? s.trimWidth : s.width;
   const size_t th = s.trimHeight ? s.trimHeight : s.height;
@@ -310,20 +361,41 @@ namespace {
 struct Meta {
   double minW, maxW;
   size_t nW;
+  // inversion size and pixel scales (the image's unless small inversion
+  // shrinks them)
+  size_t width, height;
+  double dl, dm;
 };
 
-// msgridderbase.cpp:370-446 (without small inversion)
+// msgridderbase.cpp:370-446
 Meta overallMetaData(const GridderSettings& s, const MsView& ms,
                      const ImageWeightsRef& iw) {
   double minW, maxW, maxBaseline;
   CalculateWLimits(s, ms, iw, minW, maxW, maxBaseline);
   if (minW > maxW) minW = maxW;
+  const double beamSize = 1.0 / maxBaseline;
   if (s.wLimit != 0.0) {
     maxW *= (1.0 - s.wLimit / 100.0);
     if (maxW < minW) maxW = minW;
   }
-  Meta m{minW, maxW, 0};
-  m.nW = s.nWLayers != 0 ? s.nWLayers : SuggestedWGridSize(s, minW, maxW);
+  Meta m{minW, maxW, 0, s.width, s.height, s.pixelSizeX, s.pixelSizeY};
+  if (s.smallInversion) {
+    size_t optW, optH, minWd, minHt;
+    CalculateFFTSize(m.width, m.dl, beamSize, minWd, optW);
+    CalculateFFTSize(m.height, m.dm, beamSize, minHt, optH);
+    if (optW < m.width || optH < m.height) {
+      const size_t nw = std::max(std::min(optW, m.width), size_t(32));
+      const size_t nh = std::max(std::min(optH, m.height), size_t(32));
+      m.dl = (double(m.width) * m.dl) / double(nw);
+      m.dm = (double(m.height) * m.dm) / double(nh);
+      m.width = nw;
+      m.height = nh;
+    }
+  }
+  GridderSettings sm = s;
+  sm.width = m.width;
+  sm.height = m.height;
+  m.nW = s.nWLayers != 0 ? s.nWLayers : SuggestedWGridSize(sm, minW, maxW);
   return m;
 }
 }  // namespace
@@ -335,7 +407,7 @@ template <typename T>
 InvertResult InvertMs(const GridderSettings& s, const MsView& ms,
                       const ImageWeightsRef& iw) {
   const Meta meta = overallMetaData(s, ms, iw);
-  WStackingRef<T> gridder(s.width, s.height, s.pixelSizeX, s.pixelSizeY,
+  WStackingRef<T> gridder(meta.width, meta.height, meta.dl, meta.dm,
                           s.nCpus, s.kernelSize, s.overSampling);
   gridder.SetGridMode(s.gridMode);
   if (s.lShift != 0.0 || s.mShift != 0.0)
@@ -423,10 +495,22 @@ InvertResult InvertMs(const GridderSettings& s, const MsView& ms,
   gridder.FinalizeImage(1.0 / res.totalWeight);
   const size_t n = s.width * s.height;
   std::vector<float> real(n), imag;
-  for (size_t i = 0; i != n; ++i) real[i] = float(gridder.RealImage()[i]);
+  // wsmsgridder.cpp:452-474: the inversion-size image interpolated to the
+  // image size
+  auto toImage = [&](const T* img, std::vector<float>& out) {
+    if (meta.width == s.width && meta.height == s.height) {
+      for (size_t i = 0; i != n; ++i) out[i] = float(img[i]);
+      return;
+    }
+    std::vector<double> in(meta.width * meta.height);
+    for (size_t i = 0; i != in.size(); ++i) in[i] = double(float(img[i]));
+    const std::vector<double> r = Resample(in.data(), meta.width, meta.height, s.width, s.height);
+    for (size_t i = 0; i != n; ++i) out[i] = float(r[i]);
+  };
+  toImage(gridder.RealImage(), real);
   if (s.isComplex) {
     imag.resize(n);
-    for (size_t i = 0; i != n; ++i) imag[i] = float(gridder.ImaginaryImage()[i]);
+    toImage(gridder.ImaginaryImage(), imag);
   }
   const size_t tw = s.trimWidth ? s.trimWidth : s.width;
   const size_t th = s.trimHeight ? s.trimHeight : s.height;
@@ -451,7 +535,7 @@ void PredictMs(const GridderSettings& s, const MsView& ms,
   if (!s.isComplex && imag)
     throw std::runtime_error("Imaginary specified in non-complex prediction");
   const Meta meta = overallMetaData(s, ms, iw);
-  WStackingRef<T> gridder(s.width, s.height, s.pixelSizeX, s.pixelSizeY,
+  WStackingRef<T> gridder(meta.width, meta.height, meta.dl, meta.dm,
                           s.nCpus, s.kernelSize, s.overSampling);
   gridder.SetGridMode(s.gridMode);
   const bool hasShift = s.lShift != 0.0 || s.mShift != 0.0;
@@ -471,6 +555,19 @@ void PredictMs(const GridderSettings& s, const MsView& ms,
   } else {
     re.assign(real, real + s.width * s.height);
     if (imag) im.assign(imag, imag + s.width * s.height);
+  }
+  // wsmsgridder.cpp:521-551: the image-size model decimated to the
+  // inversion size
+  if (meta.width != s.width || meta.height != s.height) {
+    auto down = [&](std::vector<float>& img) {
+      std::vector<double> in(img.begin(), img.end());
+      const std::vector<double> r =
+          Resample(in.data(), s.width, s.height, meta.width, meta.height);
+      img.assign(r.size(), 0.0f);
+      for (size_t i = 0; i != r.size(); ++i) img[i] = float(r[i]);
+    };
+    down(re);
+    if (imag) down(im);
   }
   gridder.InitializePrediction(re.data(), imag ? im.data() : nullptr);
   const size_t nCpus = std::max<size_t>(1, s.nCpus);

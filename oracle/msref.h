@@ -75,6 +75,10 @@ struct GridderSettings {
   double memory = 0;  // bytes; PrepareWLayers receives 0.6 of it
   double wLimit = 0;  // percent
   bool psfMode = false;
+  // small inversion (msgridderbase.cpp:409-439): width/height are then the
+  // image size, and the inversion runs at the size that samples the
+  // theoretical beam by two pixels, resampled to the image size
+  bool smallInversion = false;
 };
 
 struct InvertResult {
@@ -113,6 +117,17 @@ std::vector<float> trimImage(const std::vector<float>& in, size_t w, size_t h,
                              size_t tw, size_t th);
 std::vector<float> untrimImage(const float* in, size_t tw, size_t th, size_t w,
                                size_t h);
+
+// The FFT resampler WSMSGridder applies for small inversion
+// (schaapcommon::fft::Resampler, gridding/wsmsgridder.cpp:452-474, 521-551;
+// schaapcommon is not vendored in the reference tree): real-to-complex
+// transform of the in_w x in_h image, the low-frequency block shared by both
+// sizes copied into an out_w x out_h half spectrum (columns 0 .. min(in_w,
+// out_w)/2, rows 0 .. min(in_h, out_h)/2 - 1 and the last min(in_h,
+// out_h)/2 rows), complex-to-real transform of that, times
+// 1 / (in_w in_h). FP64 throughout.
+std::vector<double> Resample(const double* in, size_t inW, size_t inH, size_t outW,
+                             size_t outH);
 
 // wsmsgridder.cpp:77-128
 size_t SuggestedWGridSize(const GridderSettings& s, double minW, double maxW);

@@ -113,6 +113,7 @@ typedef struct orc_ms_settings {
   double memory;
   double w_limit;
   int psf_mode;
+  int small_inversion;
 } orc_ms_settings;
 
 typedef struct orc_invert_stats {
@@ -159,6 +160,10 @@ void orc_calculate_w_limits(const orc_ms_settings* s, size_t nrow, size_t nchan,
                             const double* uvw_m, const double* wavelengths,
                             const float* weights, void* iw, double* min_w,
                             double* max_w, double* max_baseline_uvw);
+/* FFT resampling of a real image (schaapcommon::fft::Resampler as
+ * WSMSGridder uses it for small inversion), FP64. */
+void orc_resample(const double* in, size_t in_w, size_t in_h, double* out, size_t out_w,
+                  size_t out_h);
 /* Centre-keeping trim (w x h -> tw x th) and untrim with zero fill. */
 void orc_trim_image(const float* in, size_t w, size_t h, size_t tw, size_t th,
                     float* out);

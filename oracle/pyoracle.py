@@ -51,6 +51,7 @@ class MsSettings(C.Structure):
         ("m_shift", C.c_double), ("is_complex", C.c_int),
         ("ncpus", C.c_size_t), ("memory", C.c_double),
         ("w_limit", C.c_double), ("psf_mode", C.c_int),
+        ("small_inversion", C.c_int),
     ]
 
 
@@ -116,6 +117,7 @@ def _setup(L):
         "orc_calculate_w_limits": (None, [C.POINTER(MsSettings), sz, sz, pd, pd, pf, vp, pd,
                                           pd, pd]),
         "orc_trim_image": (None, [pf, sz, sz, sz, sz, pf]),
+        "orc_resample": (None, [pd, sz, sz, pd, sz, sz]),
         "orc_untrim_image": (None, [pf, sz, sz, sz, sz, pf]),
     }
     for name, (res, args) in sig.items():
@@ -278,11 +280,12 @@ class ImageWeights:
 def ms_settings(width, height, dl, dm, *, trim=None, kernel_size=7,
                 oversampling=63, grid_mode="kb", nw=0, nw_factor=1.0,
                 shift=(0.0, 0.0), is_complex=False, ncpus=1, memory=64e9,
-                w_limit=0.0, psf=False):
+                w_limit=0.0, psf=False, small_inversion=False):
     tw, th = trim if trim else (0, 0)
     return MsSettings(width, height, tw, th, dl, dm, kernel_size, oversampling,
                       KERNEL_MODES[grid_mode], nw, nw_factor, shift[0], shift[1],
-                      int(is_complex), ncpus, memory, w_limit, int(psf))
+                      int(is_complex), ncpus, memory, w_limit, int(psf),
+                      int(small_inversion))
 
 
 def ms_invert(settings, uvw_m, wavelengths, data, weights=None, image_weights=None,
@@ -390,4 +393,14 @@ def untrim_image(image, width, height):
     th, tw = img.shape
     out = np.zeros((height, width), np.float32)
     lib().orc_untrim_image(_p(img, C.c_float), tw, th, width, height, _p(out, C.c_float))
+    return out
+
+
+def resample(image, out_width, out_height):
+    """FFT resampling of a real [H, W] image to [out_height, out_width] (the
+    small-inversion resampler), FP64."""
+    img = _c(image, np.float64)
+    h, w = img.shape
+    out = np.zeros((out_height, out_width), np.float64)
+    lib().orc_resample(_p(img, C.c_double), w, h, _p(out, C.c_double), out_width, out_height)
     return out

@@ -5,6 +5,7 @@
 //   host_tests cpu   -- tests that need no GPU
 //   host_tests gpu   -- parity tests on the GPU
 #include "../../wsclean_amd/host/griddingtaskmanager.h"
+#include "../../wsclean_amd/host/hipdirectftgridder.h"
 #include "../../wsclean_amd/host/hipwsmsgridder.h"
 #include "../../oracle/oracle.h"
 
@@ -124,6 +125,7 @@ orc_ms_settings oracleSettings(const Settings& s, bool complexPol, double ls = 0
   o.is_complex = complexPol;
   o.ncpus = s.threadCount;
   o.memory = double(s.memory);
+  o.small_inversion = s.smallInversion ? 1 : 0;
   return o;
 }
 
@@ -351,6 +353,163 @@ void gpuTests() {
     s.hipMaxLayerMemory = 96.0 * 96 * 8 * 5;  // five layers per pass
     GriddingResult many = invertTask(s, p, w, false);
     CHECK_LT(relL2(many.images[0].Data(), one.images[0].Data(), 80 * 80), 1e-6);
+  });
+  // Small inversion (msgridderbase.cpp:409-439; wsmsgridder.cpp:452-484,
+  // 516-551): 0.001 rad pixels on a 256^2 image of baselines out to 0.12 /
+  // pixel sample the beam by ~8 pixels, so the inversion runs at 64^2 and is
+  // resampled to 256^2 on the device; compared with the oracle's WSMSGridder
+  // with the same small inversion (FP64 FFT resampler).
+  run("small_inversion_matches_oracle", [&] {
+    Observation so;
+    so.pixel = 0.001;
+    so.width = 256;
+    std::mt19937_64 rng(99);
+    std::uniform_real_distribution<double> uni(-1.0, 1.0);
+    const double umax = 0.12 / so.pixel * so.wl[so.nchan - 1];
+    for (size_t r = 0; r != so.nrow; ++r) {
+      so.uvw[r * 3] = uni(rng) * umax;
+      so.uvw[r * 3 + 1] = uni(rng) * umax;
+    }
+    Settings s = baseSettings(so);
+    s.trimmedImageWidth = s.trimmedImageHeight = 200;
+    s.smallInversion = true;
+    MemoryMSProvider p = so.provider(false);
+    auto w = hostWeights(so, s, p);
+    GriddingResult r = invertTask(s, p, w, false);
+    orc_ms_settings os = oracleSettings(s, false);
+    void* iw = oracleWeights(so, s);
+    std::vector<float> ref(200 * 200);
+    orc_invert_stats st{};
+    CHECK(orc_ms_invert(0, &os, so.nrow, so.nchan, so.uvw.data(), so.wl.data(),
+                        reinterpret_cast<const float*>(so.xx.data()), so.weights.data(), iw,
+                        ref.data(), nullptr, &st) == 0);
+    CHECK(r.images.size() == 1 && r.images[0].Size() == 200 * 200);
+    const double e = relL2(r.images[0].Data(), ref.data(), ref.size());
+    std::printf("  small-inversion image rel L2 vs oracle = %.3g (nW %zu)\n", e, st.nw);
+    CHECK_LT(e, 1e-5);
+    CHECK(r.actualWGridSize == st.nw);
+    // predict: the 200^2 model untrimmed to 256^2 and decimated
+    std::normal_distribution<float> gauss;
+    Image model(200, 200);
+    for (size_t i = 0; i != model.Size(); ++i) model[i] = gauss(rng);
+    std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
+    GriddingTask t;
+    t.operation = GriddingTask::Predict;
+    t.imageWeights = w;
+    t.msList = {&p};
+    t.modelImages.push_back(model);
+    tm->RunDirect(std::move(t));
+    std::vector<std::complex<float>> vref(so.nrow * so.nchan, std::complex<float>(0, 0));
+    CHECK(orc_ms_predict(0, &os, so.nrow, so.nchan, so.uvw.data(), so.wl.data(),
+                         so.weights.data(), iw, model.Data(), nullptr,
+                         reinterpret_cast<float*>(vref.data()), &st) == 0);
+    orc_iw_destroy(iw);
+    double num = 0, den = 0;
+    for (size_t i = 0; i != vref.size(); ++i) {
+      num += std::norm(std::complex<double>(p.Model()[i]) - std::complex<double>(vref[i]));
+      den += std::norm(std::complex<double>(vref[i]));
+    }
+    std::printf("  small-inversion predict rel L2 vs oracle = %.3g\n", std::sqrt(num / den));
+    CHECK_LT(std::sqrt(num / den), 1e-5);
+  });
+  // Several measurement sets: with one band they are staged together (the
+  // device path) and equal one set holding all rows; with different bands
+  // they go through the host-pointer protocol.
+  run("measurement_sets_staged_together_and_by_band", [&] {
+    Settings s = baseSettings(o);
+    s.nWLayers = 12;
+    const size_t half = o.nrow / 2;
+    auto part = [&](size_t r0, size_t r1, std::vector<double> wl) {
+      std::vector<double> uvw(o.uvw.begin() + r0 * 3, o.uvw.begin() + r1 * 3);
+      std::vector<std::complex<float>> d(o.xx.begin() + r0 * o.nchan, o.xx.begin() + r1 * o.nchan);
+      std::vector<float> wt(o.weights.begin() + r0 * o.nchan, o.weights.begin() + r1 * o.nchan);
+      std::vector<double> tm(o.times.begin() + r0, o.times.begin() + r1);
+      return MemoryMSProvider(uvw, wl, d, wt, tm);
+    };
+    MemoryMSProvider all = o.provider(false), a = part(0, half, o.wl), b = part(half, o.nrow, o.wl);
+    auto w = hostWeights(o, s, all);
+    GriddingResult one = invertTask(s, all, w, false);
+    std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
+    GriddingTask t;
+    t.imageWeights = w;
+    t.msList = {&a, &b};
+    GriddingResult two = tm->RunDirect(std::move(t));
+    const double e = relL2(two.images[0].Data(), one.images[0].Data(), 80 * 80);
+    std::printf("  two sets, one band vs one set: rel L2 = %.3g\n", e);
+    CHECK_LT(e, 1e-6);
+    CHECK(two.griddedVisibilityCount == one.griddedVisibilityCount);
+    // a second set on another band: the host-pointer protocol
+    std::vector<double> wl2 = o.wl;
+    for (double& x : wl2) x *= 1.05;
+    MemoryMSProvider c = part(half, o.nrow, wl2);
+    HipWSMSGridder g(s, Resources(1, int64_t(1) << 32));
+    g.SetImageWidth(s.paddedImageWidth);
+    g.SetImageHeight(s.paddedImageHeight);
+    g.SetTrimSize(80, 80);
+    g.SetImageWeights(w.get());
+    g.AddMeasurementSet(&a);
+    g.AddMeasurementSet(&c);
+    g.Invert();
+    CHECK(!g.UsedStagedPath());
+    std::vector<Image> im = g.ResultImages();
+    double peak = 0;
+    for (size_t i = 0; i != im[0].Size(); ++i) peak = std::max(peak, double(std::fabs(im[0][i])));
+    CHECK(peak > 0.0 && std::isfinite(peak));
+  });
+  // The direct FT gridder (GridderType::DirectFT) against the oracle's FP64
+  // DFT of the same weighted samples, at the trimmed size; its predict
+  // against the oracle's DFT predict.
+  run("direct_ft_gridder_matches_oracle_dft", [&] {
+    Settings s = baseSettings(o);
+    s.gridderType = GridderType::DirectFT;
+    s.trimmedImageWidth = s.trimmedImageHeight = 48;
+    MemoryMSProvider p = o.provider(false);
+    auto w = hostWeights(o, s, p);
+    GriddingResult r = invertTask(s, p, w, false, 0.004, -0.003);
+    void* iw = oracleWeights(o, s);
+    std::vector<float> imw(o.nrow * o.nchan);
+    orc_iw_lookup(iw, o.nrow, o.nchan, o.uvw.data(), o.wl.data(), imw.data());
+    orc_iw_destroy(iw);
+    std::vector<double> uvwl;
+    std::vector<std::complex<float>> vw;
+    double sumw = 0;
+    for (size_t row = 0; row != o.nrow; ++row)
+      for (size_t ch = 0; ch != o.nchan; ++ch) {
+        const size_t i = row * o.nchan + ch;
+        const float cw = o.weights[i] * imw[i];
+        if (cw != 0.0f) sumw += double(cw);
+        for (int k = 0; k != 3; ++k) uvwl.push_back(o.uvw[row * 3 + k] / o.wl[ch]);
+        vw.push_back(o.xx[i] * cw);
+      }
+    std::vector<double> ref(48 * 48);
+    orc_dft_invert(48, 48, s.pixelScaleX, s.pixelScaleY, 0.004, -0.003, vw.size(), uvwl.data(),
+                   reinterpret_cast<const float*>(vw.data()), 1.0 / sumw, ref.data(), 4);
+    std::vector<float> reff(ref.begin(), ref.end());
+    const double e = relL2(r.images[0].Data(), reff.data(), reff.size());
+    std::printf("  direct FT gridder image rel L2 vs oracle DFT = %.3g\n", e);
+    CHECK_LT(e, 1e-6);
+    std::mt19937 rng(3);
+    std::normal_distribution<float> gauss;
+    Image model(48, 48);
+    for (size_t i = 0; i != model.Size(); i += 7) model[i] = gauss(rng);
+    std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
+    GriddingTask t;
+    t.operation = GriddingTask::Predict;
+    t.imageWeights = w;
+    t.msList = {&p};
+    t.modelImages.push_back(model);
+    tm->RunDirect(std::move(t));
+    std::vector<double> md(model.Data(), model.Data() + model.Size());
+    std::vector<std::complex<double>> vref(vw.size());
+    orc_dft_predict(48, 48, s.pixelScaleX, s.pixelScaleY, 0.0, 0.0, vw.size(), uvwl.data(),
+                    md.data(), reinterpret_cast<double*>(vref.data()), 4);
+    double num = 0, den = 0;
+    for (size_t i = 0; i != vref.size(); ++i) {
+      num += std::norm(std::complex<double>(p.Model()[i]) - vref[i]);
+      den += std::norm(vref[i]);
+    }
+    std::printf("  direct FT gridder predict rel L2 vs oracle DFT = %.3g\n", std::sqrt(num / den));
+    CHECK_LT(std::sqrt(num / den), 1e-6);
   });
   run("psf_peak_is_one", [&] {
     Settings s = baseSettings(o);

@@ -33,4 +33,4 @@ def test_host_layer_cpu():
 @pytest.mark.gpu
 def test_host_layer_gpu():
     out = _run("gpu")
-    assert out.count("PASS") == 7
+    assert out.count("PASS") == 10

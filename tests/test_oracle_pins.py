@@ -129,3 +129,35 @@ def test_trim_keeps_the_phase_centre_pixel(oracle):
         mask = np.ones((h, w), bool)
         mask[sy:sy + th, sx:sx + tw] = False
         assert not un[mask].any()
+
+
+# ---- the small-inversion resampler (schaapcommon::fft::Resampler, not
+# vendored in the reference tree; its published algorithm restated in
+# oracle/msref.cpp::Resample). Pinned by closed forms: a constant stays
+# constant, and an image that is a trigonometric polynomial with frequencies
+# below both grids' Nyquist frequencies comes out as the same function
+# sampled on the new grid. Content on a Nyquist line (where the
+# real-to-complex copy keeps one of the two half-weight terms) is not
+# pinned by any reference vector or closed form.
+
+def _trig_image(terms, w, h):
+    y, x = np.mgrid[0:h, 0:w]
+    img = np.zeros((h, w))
+    for a, fx, fy, ph in terms:
+        img += a * np.cos(2 * np.pi * (fx * x / w + fy * y / h) + ph)
+    return img
+
+
+@pytest.mark.parametrize("in_w,in_h,out_w,out_h", [(32, 32, 64, 64), (64, 48, 40, 32),
+                                                   (36, 40, 96, 72), (128, 128, 60, 100)])
+def test_resample_constant_and_band_limited(oracle, in_w, in_h, out_w, out_h):
+    out = oracle.resample(np.full((in_h, in_w), 2.5), out_w, out_h)
+    assert np.max(np.abs(out - 2.5)) < 1e-13
+    rng = np.random.default_rng(in_w * out_w + in_h)
+    lim_x, lim_y = min(in_w, out_w) // 2, min(in_h, out_h) // 2
+    terms = [(rng.uniform(0.2, 2.0), int(rng.integers(-lim_x + 1, lim_x)),
+              int(rng.integers(-lim_y + 1, lim_y)), rng.uniform(0, 2 * np.pi)) for _ in range(8)]
+    terms.append((1.0, lim_x - 1, -(lim_y - 1), 0.3))  # the highest kept frequencies
+    got = oracle.resample(_trig_image(terms, in_w, in_h), out_w, out_h)
+    want = _trig_image(terms, out_w, out_h)
+    assert np.max(np.abs(got - want)) < 1e-11 * sum(abs(t[0]) for t in terms)

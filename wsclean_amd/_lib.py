@@ -48,6 +48,17 @@ class HwsStats(C.Structure):
         ("tiles_needed", C.c_uint64 * 2), ("tile_rows_total", C.c_uint64),
         ("tile_rows_live", C.c_uint64 * 2), ("tile_cols_total", C.c_uint64),
         ("tile_cols_needed", C.c_uint64 * 2),
+        ("total_weight", C.c_double * 4), ("visibility_weight_sum", C.c_double * 4),
+        ("max_gridded_weight", C.c_double * 4), ("gridded_visibility_count", C.c_uint64 * 4),
+    ]
+
+
+class HwsWeightsConfig(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("briggs_robustness", C.c_double),
+        ("width", C.c_size_t), ("height", C.c_size_t),
+        ("pixel_scale_x", C.c_double), ("pixel_scale_y", C.c_double),
+        ("weights_as_taper", C.c_int), ("super_weight", C.c_double), ("device", C.c_int),
     ]
 
 
@@ -96,12 +107,31 @@ SIGNATURES = {
                                             C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "hws_graph_capture_predict_batch": (_i, [C.POINTER(_vp), _sz, C.POINTER(_vp),
                                              C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "hws_weights_create": (_i, [C.POINTER(HwsWeightsConfig), C.POINTER(_vp)]),
+    "hws_weights_destroy": (None, [_vp]),
+    "hws_weights_grid_device": (_i, [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp]),
+    "hws_weights_finish": (_i, [_vp, _vp]),
+    "hws_weights_shape": (_i, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
+    "hws_weights_get_grid": (_i, [_vp, _pd, _pd]),
+    "hws_set_weighting": (_i, [_vp, _vp, _vp, _i]),
+    "hws_stage_begin": (_i, [_vp, _sz, _sz, _pd, _i]),
+    "hws_stage_rows": (_i, [_vp, _sz, _pd, _pf]),
+    "hws_stage_end": (_i, [_vp]),
+    "hws_invert_staged": (_i, [_vp, _d, _sz, _sz, _sz, _sz, _pf, _pf]),
+    "hws_predict_staged": (_i, [_vp, _pf, _pf, _sz, _sz, _sz, _sz, _pf]),
+    "hws_finalize_image_resampled": (_i, [_vp, _d, _sz, _sz, _sz, _sz, _pf, _pf]),
+    "hws_initialize_prediction_resampled": (_i, [_vp, _pf, _pf, _sz, _sz, _sz, _sz]),
     "hws_graph_launch": (_i, [_vp, _vp]),
     "hws_graph_destroy": (None, [_vp]),
     "hws_direct_invert_device": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _sz, _sz, _vp, _vp,
                                       _vp, _sz, _d, _vp]),
     "hws_direct_predict_device": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _vp, _sz, _sz, _vp,
                                        _vp, _vp, _sz]),
+    "hws_direct_invert": (_i, [_i, _sz, _sz, _d, _d, _d, _d, _sz, _sz, _pd, _pd, _pf, _d, _pd]),
+    "hws_direct_predict": (_i, [_i, _sz, _sz, _d, _d, _d, _d, _pf, _sz, _sz, _pd, _pd, _pf]),
+    "hws_resample_device": (_i, [_i, _vp, _vp, _sz, _sz, _vp, _sz, _sz, _sz]),
+    "hws_trim_device": (_i, [_i, _vp, _vp, _sz, _sz, _vp, _sz, _sz, _sz]),
+    "hws_untrim_device": (_i, [_i, _vp, _vp, _sz, _sz, _vp, _sz, _sz, _sz]),
     "hws_direct_predict_device_noalloc": (_i, [_i, _vp, _sz, _sz, _d, _d, _d, _d, _vp, _sz, _sz,
                                                _vp, _vp, _vp, _sz]),
 }

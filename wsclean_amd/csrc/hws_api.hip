@@ -448,6 +448,7 @@ void hws_destroy(hws_gridder* g) {
   freeDev(g->dStageWl);
   freeDev(g->dStageVis);
   freeDev(g->dStageOut);
+  freeStaging(g);
   freeFftPlan(g->rowPlan);
   freeFftPlan(g->colPlan);
   Plan& pl = g->plan;
@@ -456,6 +457,9 @@ void hws_destroy(hws_gridder* g) {
   freeDev(pl.counters);
   freeDev(pl.queue);
   freeDev(pl.rot);
+  freeDev(pl.imgw);
+  freeDev(g->dWeightSums);
+  freeDev(g->dWeightPartials);
   freeDev(pl.rowKeys);
   freeDev(pl.rowKeysAlt);
   freeDev(pl.rowOrder);
@@ -767,6 +771,7 @@ hws_status hws_plan_device(hws_gridder* g, size_t nrow, size_t nchan,
   } else {
     HWS_TRY(buildPlan(g, nrow, nchan, d_uvw_m, d_wavelengths));
   }
+  HWS_TRY(planWeights(g));
   HWS_TRY(record(g, 1));
   g->stats = hws_stats{};
   g->lastOp = 0;
@@ -781,6 +786,12 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
   if (!d_real || (g->plan.nrow && !d_vis)) return fail(g, HWS_ERR_INVALID_ARGUMENT, "null data pointer");
   (void)hipSetDevice(g->cfg.device);
   Plan& pl = g->plan;
+  // weighting attached after the plan: its per-sample weights and sums first
+  if (g->weights && !pl.weighted) {
+    if (g->capturing)
+      return fail(g, HWS_ERR_BAD_STATE, "plan again after attaching weights before a capture");
+    HWS_TRY(planWeights(g));
+  }
   HWS_TRY(record(g, 2));
   // the batched forms' first projection writes the accumulators outright
   const bool plainForm = g->cfg.precision == HWS_PRECISION_PLAIN_FP32;
@@ -939,7 +950,41 @@ hws_status hws_get_stats(const hws_gridder* cg, hws_stats* out) {
     for (const auto& kp : g->plan.kinds)
       if (kp.active) s.chunks += kp.nWorkItems;
   }
+  if (g->weights && g->weightSumsValid) {
+    double w[kMaxImages * 4] = {};
+    HWS_TRY(checkHip(g, hipMemcpyAsync(w, g->dWeightSums, sizeof(w), hipMemcpyDeviceToHost,
+                                       g->stream), "weight sums"));
+    HWS_TRY(checkHip(g, hipStreamSynchronize(g->stream), "weight sums"));
+    for (int i = 0; i < g->nImg; ++i) {
+      s.total_weight[i] = w[i * 4];
+      s.visibility_weight_sum[i] = w[i * 4 + 1];
+      s.max_gridded_weight[i] = w[i * 4 + 2];
+      s.gridded_visibility_count[i] = uint64_t(w[i * 4 + 3]);
+    }
+  }
   *out = s;
+  return HWS_OK;
+}
+
+hws_status hws_set_weighting(hws_gridder* g, const hws_weights* weights,
+                             const float* d_vis_weights, int visibility_weighting_mode) {
+  if (!g) return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "null context");
+  if (visibility_weighting_mode < 0 || visibility_weighting_mode > 2)
+    return fail(g, HWS_ERR_INVALID_ARGUMENT, "unknown visibility weighting mode");
+  if (weights && !weights->finished)
+    return fail(g, HWS_ERR_BAD_STATE,
+                "imaging weights are not finished (call hws_weights_finish first)");
+  if (weights && weights->device != g->cfg.device)
+    return fail(g, HWS_ERR_INVALID_ARGUMENT, "imaging weights live on another device");
+  if (!weights && d_vis_weights)
+    return fail(g, HWS_ERR_INVALID_ARGUMENT, "visibility weights need imaging weights");
+  g->weights = weights;
+  g->dVisWeights = weights ? d_vis_weights : nullptr;
+  g->visWeightMode = weights ? visibility_weighting_mode : 0;
+  // the imaging weights of the samples and the weight sums are computed by
+  // the next hws_plan_device, or by the next invert of the current plan
+  g->plan.weighted = false;
+  g->weightSumsValid = false;
   return HWS_OK;
 }
 
@@ -954,6 +999,15 @@ struct hws_graph {
 }  // extern "C"
 
 namespace {
+
+// The per-sample imaging weights and weight sums of an attached weighting,
+// computed (and their buffers allocated) before a capture starts.
+hws_status weightsBeforeCapture(hws_gridder* g) {
+  if (!g->weights || g->plan.weighted) return HWS_OK;
+  (void)hipSetDevice(g->cfg.device);
+  HWS_TRY(planWeights(g));
+  return checkHip(g, hipStreamSynchronize(g->stream), "weights before capture");
+}
 
 hws_status graphChecks(hws_gridder* g, hws_graph** out) {
   if (!g || !out) return fail(g, HWS_ERR_INVALID_ARGUMENT, "null argument");
@@ -1087,6 +1141,7 @@ extern "C" {
 hws_status hws_graph_capture_invert(hws_gridder* g, const float* d_vis, double mult,
                                     float* d_real, float* d_imag, hws_graph** out) {
   HWS_TRY(graphChecks(g, out));
+  HWS_TRY(weightsBeforeCapture(g));  // allocations cannot be captured
   return captureBatch(
       &g, 1, [&](size_t) { return hws_invert_planned_device(g, d_vis, mult, d_real, d_imag); },
       out);
@@ -1111,6 +1166,7 @@ hws_status hws_graph_capture_invert_batch(hws_gridder* const* gs, size_t n,
     return fail(gs && n ? gs[0] : nullptr, HWS_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   HWS_TRY(batchChecks(gs, n, false));
+  for (size_t i = 0; i < n; ++i) HWS_TRY(weightsBeforeCapture(gs[i]));
   return captureBatch(gs, n, [&](size_t i) {
     return hws_invert_planned_device(gs[i], d_vis[i], mult[i], d_real[i],
                                      d_imag ? d_imag[i] : nullptr);

@@ -376,3 +376,88 @@ extern "C" hws_status hws_direct_predict_device_noalloc(
                              l_shift, m_shift, d_image, nrow, nchan, d_uvw_m, d_wavelengths,
                              d_vis_out, vis_stride, counts, terms);
 }
+
+// ---- host-pointer forms (for host callers such as the direct-FT plugin) ----
+
+namespace hws {
+namespace {
+
+struct DeviceBuffers {
+  void* p[5] = {};
+  ~DeviceBuffers() {
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+  }
+};
+
+hws_status devUpload(void** dst, const void* src, size_t bytes, hipStream_t stream) {
+  if (hipMalloc(dst, bytes ? bytes : 1) != hipSuccess)
+    return setLastError(nullptr, HWS_ERR_OUT_OF_MEMORY, "direct transform buffers");
+  if (src && bytes)
+    return checkHip(nullptr, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, stream),
+                    "direct transform upload");
+  return HWS_OK;
+}
+
+}  // namespace
+}  // namespace hws
+
+extern "C" hws_status hws_direct_invert(int device, size_t width, size_t height, double dl,
+                                        double dm, double l_shift, double m_shift, size_t nrow,
+                                        size_t nchan, const double* uvw_m,
+                                        const double* wavelengths, const float* vis,
+                                        double multiplication_factor, double* image) {
+  const hws_status valid = validateGeometry(width, height, dl, dm, nrow, nchan, 1);
+  if (valid != HWS_OK) return valid;
+  if (!image || (nrow && (!uvw_m || !wavelengths || !vis)))
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "null data pointer");
+  DeviceGuard guard(device);
+  if (!guard.ok) return setLastError(nullptr, HWS_ERR_HIP, "hipSetDevice failed");
+  DeviceBuffers b;
+  hipStream_t stream = nullptr;
+  hws_status st = devUpload(&b.p[0], uvw_m, nrow * 3 * sizeof(double), stream);
+  if (st == HWS_OK) st = devUpload(&b.p[1], wavelengths, nchan * sizeof(double), stream);
+  if (st == HWS_OK) st = devUpload(&b.p[2], vis, nrow * nchan * 2 * sizeof(float), stream);
+  if (st == HWS_OK) st = devUpload(&b.p[3], nullptr, width * height * sizeof(double), stream);
+  if (st == HWS_OK)
+    st = hws_direct_invert_device(device, stream, width, height, dl, dm, l_shift, m_shift, nrow,
+                                  nchan, static_cast<const double*>(b.p[0]),
+                                  static_cast<const double*>(b.p[1]),
+                                  static_cast<const float*>(b.p[2]), 1, multiplication_factor,
+                                  static_cast<double*>(b.p[3]));
+  if (st == HWS_OK)
+    st = checkHip(nullptr, hipMemcpy(image, b.p[3], width * height * sizeof(double),
+                                     hipMemcpyDeviceToHost),
+                  "direct transform download");
+  return st;
+}
+
+extern "C" hws_status hws_direct_predict(int device, size_t width, size_t height, double dl,
+                                         double dm, double l_shift, double m_shift,
+                                         const float* image, size_t nrow, size_t nchan,
+                                         const double* uvw_m, const double* wavelengths,
+                                         float* vis_out) {
+  const hws_status valid = validateGeometry(width, height, dl, dm, nrow, nchan, 1);
+  if (valid != HWS_OK) return valid;
+  if (!image || (nrow && (!uvw_m || !wavelengths || !vis_out)))
+    return setLastError(nullptr, HWS_ERR_INVALID_ARGUMENT, "null data pointer");
+  DeviceGuard guard(device);
+  if (!guard.ok) return setLastError(nullptr, HWS_ERR_HIP, "hipSetDevice failed");
+  DeviceBuffers b;
+  hipStream_t stream = nullptr;
+  hws_status st = devUpload(&b.p[0], uvw_m, nrow * 3 * sizeof(double), stream);
+  if (st == HWS_OK) st = devUpload(&b.p[1], wavelengths, nchan * sizeof(double), stream);
+  if (st == HWS_OK) st = devUpload(&b.p[2], image, width * height * sizeof(float), stream);
+  if (st == HWS_OK) st = devUpload(&b.p[3], nullptr, nrow * nchan * 2 * sizeof(float), stream);
+  if (st == HWS_OK)
+    st = hws_direct_predict_device(device, stream, width, height, dl, dm, l_shift, m_shift,
+                                   static_cast<const float*>(b.p[2]), nrow, nchan,
+                                   static_cast<const double*>(b.p[0]),
+                                   static_cast<const double*>(b.p[1]),
+                                   static_cast<float*>(b.p[3]), 1);
+  if (st == HWS_OK)
+    st = checkHip(nullptr, hipMemcpy(vis_out, b.p[3], nrow * nchan * 2 * sizeof(float),
+                                     hipMemcpyDeviceToHost),
+                  "direct transform download");
+  return st;
+}

@@ -107,6 +107,42 @@ __device__ __forceinline__ void turnSinCos(float f, float& s, float& c) {
 }
 #endif
 
+// The imaging-weight grid of an ImageWeights (structures/imageweights.h:
+// 73-94): `width` x `height`/2 doubles over the v >= 0 half of the uv plane.
+struct WeightGridArgs {
+  const double* grid;
+  int width, height;
+  double dl, dm;
+};
+
+#ifdef __HIPCC__
+// Cell of (u, v) (wavelengths) in the weight grid, or -1 outside it:
+// uvToXY + isWithinLimits (structures/imageweights.h:73-94). The products and
+// the sum are rounded one by one in the reference's order (no contraction
+// into an FMA), so a sample on a cell boundary lands in the reference's cell.
+__device__ inline long long weightCell(const WeightGridArgs& w, double u, double v) {
+  if (v < 0.0) {
+    u = -u;
+    v = -v;
+  }
+  const double xf = __dadd_rn(__dmul_rn(__dmul_rn(u, double(w.width)), w.dl),
+                              double(w.width / 2));
+  const double yf = __dmul_rn(__dmul_rn(v, double(w.height)), w.dm);
+  const double fx = floor(xf), fy = floor(yf);
+  // bounded before the integer conversion (the reference converts unchecked)
+  if (!(fx >= 0.0) || !(fx < double(w.width)) || !(fy < double(w.height / 2)) || !(fy >= 0.0))
+    return -1;
+  return (long long)fx + (long long)fy * w.width;
+}
+
+// ImageWeights::GetWeight (structures/imageweights.h:34): the grid value at
+// (u, v), 0 outside the grid.
+__device__ inline double imagingWeightAt(const WeightGridArgs& w, double u, double v) {
+  const long long c = weightCell(w, u, v);
+  return c >= 0 ? w.grid[c] : 0.0;
+}
+#endif
+
 __host__ __device__ inline double layerToW(const LayerMap& m, int layer) {
   if (m.nW == 1) return 0.0;
   if (m.isComplex) return layer * (m.maxW + m.maxW) / (m.nW - 1) - m.maxW;
@@ -195,9 +231,31 @@ struct Plan {
          rowOrderAltCapacity = 0;
   void* sortTemp = nullptr;  // radix sort scratch
   size_t sortTempCapacity = 0;
+  // weighted contexts: the imaging weight of every sample (float, as
+  // MSGridderBase keeps it, gridding/msgridderbase.h:502)
+  float* imgw = nullptr;
+  size_t imgwCapacity = 0;
+  bool weighted = false;  // imgw holds this plan's samples
 };
 
 }  // namespace hws
+
+// An ImageWeights on one device (structures/imageweights.{h,cpp}): the
+// weight grid, the running total of gridded weights, and the scratch of the
+// fixed-order reductions.
+struct hws_weights {
+  int mode = 0;  // hws_weight_mode
+  double robust = 0.0;
+  int weightsAsTaper = 0;
+  int device = 0;
+  int width = 0, height = 0;  // the grid holds width x height / 2 cells
+  double dl = 0.0, dm = 0.0;
+  double* dGrid = nullptr;
+  double* dTotal = nullptr;     // sum of the gridded weights (one double)
+  double* dPartials = nullptr;  // per-block partial sums
+  size_t partialCapacity = 0;
+  bool finished = false;
+};
 
 struct hws_gridder {
   hws_config cfg{};
@@ -254,6 +312,30 @@ struct hws_gridder {
   float* dStageOut = nullptr;
   size_t stageRows = 0, stageVis = 0, stageWl = 0, stageOut = 0;
 
+  // imaging weights (hws_set_weighting): the weight grid, the visibility
+  // weights [sample][image] (null: 1) and their weighting mode, and the
+  // per-image sums of the last plan: [image][0 total weight, 1 visibility
+  // weight sum, 2 max gridded weight, 3 gridded count]
+  const hws_weights* weights = nullptr;
+  const float* dVisWeights = nullptr;
+  int visWeightMode = 0;
+  double* dWeightSums = nullptr;  // [kMaxImages][4]
+  double* dWeightPartials = nullptr;
+  size_t weightPartialCapacity = 0;
+  bool weightSumsValid = false;
+
+  // staged host rows (hws_stage_*): device copies of the rows' uvw and
+  // visibilities, and two pinned host buffers the uploads go through
+  double* dStagedUvw = nullptr;
+  double* dStagedWl = nullptr;
+  float* dStagedVis = nullptr;
+  size_t stagedUvwCap = 0, stagedWlCap = 0, stagedVisCap = 0;
+  unsigned char* pinned[2] = {nullptr, nullptr};
+  hipEvent_t pinnedFree[2] = {nullptr, nullptr};
+  int pinnedNext = 0;
+  size_t stagedRows = 0, stagedMaxRows = 0, stagedNchan = 0;
+  bool stagedWithVis = false, staging = false, staged = false;
+
   // timing
   hipEvent_t ev[8] = {};
   hws_stats stats{};
@@ -271,6 +353,12 @@ hws_status setLastError(hws_gridder* g, hws_status s, const std::string& msg);
 hws_status checkHip(hws_gridder* g, hipError_t e, const char* what);
 
 hws_status launchFill(hws_gridder* g, void* ptr, uint32_t value, size_t n32);
+// Releases the staging buffers of hws_stage_* (hws_stage.hip).
+void freeStaging(hws_gridder* g);
+// The imaging weight of every planned sample and the per-image weight sums
+// of the attached weighting (hws_set_weighting), on the context's stream.
+hws_status planWeights(hws_gridder* g);
+WeightGridArgs weightGridArgs(const hws_weights* w);
 hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl);
 // sparseLayers: tiles without samples are left unwritten (not cleared); the

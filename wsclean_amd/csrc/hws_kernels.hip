@@ -755,7 +755,35 @@ struct GridArgs {
   size_t nchan;
   uint32_t chunk;    // records per work item
   const float2* rot;  // rotating contexts: the gridding phasor of every sample
+  // weighted contexts (hws_set_weighting): the imaging weight of every
+  // sample (null: the visibilities are gridded as given), the visibility
+  // weights [sample][image] (null: 1) and the visibility weighting mode
+  const float* imgw;
+  const float* visw;
+  int visWeightMode;
 };
+
+// The weight readAndWeightVisibilities multiplies a visibility by
+// (gridding/msgridderbase.cpp:567-589, 640-659): the visibility weight after
+// the visibility weighting mode (as given, squared, or 1 where non-zero)
+// times the sample's imaging weight, a float product as the reference forms
+// it.
+__device__ __forceinline__ float visibilityWeight(const float* __restrict__ imgw,
+                                                  const float* __restrict__ visw, int mode,
+                                                  size_t sample, int nImgTotal, int img) {
+  float w = visw ? visw[sample * size_t(nImgTotal) + size_t(img)] : 1.0f;
+  if (mode == 1)
+    w = w * w;
+  else if (mode == 2)
+    w = w != 0.0f ? 1.0f : 0.0f;
+  return w * imgw[sample];
+}
+
+__device__ __forceinline__ float2 weighted(const GridArgs& a, float2 v, uint32_t idx, int img) {
+  if (!a.imgw) return v;
+  const float w = visibilityWeight(a.imgw, a.visw, a.visWeightMode, idx, a.nImgTotal, img);
+  return make_float2(v.x * w, v.y * w);
+}
 
 // The phase-centre rotation of a sample as a unit phasor, once per sample
 // for all its images: MSGridderBase::rotateVisibilities
@@ -918,7 +946,8 @@ gridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
           const float2 ph = a.ga.rotate ? a.rot[idx] : make_float2(1.0f, 0.0f);
 #pragma unroll
           for (int gi = 0; gi < NIMG; ++gi) {
-            float2 v = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+            float2 v = weighted(a, vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]], idx,
+                                a.imgs.idx[gi]);
             if (a.ga.rotate) v = rotateBy(v, ph);
             if (conj) v.y = -v.y;
             myVis[lane * NIMG + gi] = v;
@@ -1176,7 +1205,8 @@ gridCellsKernel(GridArgs a, const uint32_t* __restrict__ binStart,
         if (ROT) ph = a.rot[idx];
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
-          float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+          float2 x = weighted(a, vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]], idx,
+                              a.imgs.idx[gi]);
           if (ROT) x = rotateBy(x, ph);
           if (conj) x.y = -x.y;
           v[gi] = x;
@@ -1376,7 +1406,8 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
         if (ROT) ph = a.rot[idx];
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
-          float2 x = vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]];
+          float2 x = weighted(a, vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]], idx,
+                              a.imgs.idx[gi]);
           if (ROT) x = rotateBy(x, ph);
           if (conj) x.y = -x.y;
           v[gi] = x;
@@ -1731,6 +1762,9 @@ struct PlainArgs {
   int ch;
   size_t nrow, nchan;
   int blocked;
+  const float* imgw;  // as GridArgs
+  const float* visw;
+  int visWeightMode;
 };
 
 struct PlainPlacement {
@@ -1798,6 +1832,11 @@ gridPlainKernel(PlainArgs a, const double* __restrict__ uvw,
   const PlainPlacement p = placePlain(a, uvw, wl, r, -1.0f);
   if (!p.ok) return;
   float2 v = vis[(r * a.nchan + a.ch) * a.nImgTotal + a.img];
+  if (a.imgw) {
+    const float w = visibilityWeight(a.imgw, a.visw, a.visWeightMode, r * a.nchan + a.ch,
+                                     a.nImgTotal, a.img);
+    v = make_float2(v.x * w, v.y * w);
+  }
   if (a.rotate) v = make_float2(v.x * p.rotRe - v.y * p.rotIm, v.x * p.rotIm + v.y * p.rotRe);
   if (p.conj) v.y = -v.y;
   const size_t HW = size_t(a.W) * a.H;
@@ -1993,14 +2032,18 @@ __global__ void __launch_bounds__(kThreads)
 finalizeKernel(int W, int H, const float* __restrict__ sum,
                const float* __restrict__ comp, double mult,
                const float* __restrict__ corrX, const float* __restrict__ corrY,
-               double norm, int correct, float* __restrict__ out) {
+               double norm, int correct, const double* __restrict__ weightSums,
+               float* __restrict__ out) {
   const size_t HW = size_t(W) * H;
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= HW) return;
   const size_t img = blockIdx.y;
   const int x = int(i % W), y = int(i / W);
   float v = sum[img * HW + i] - comp[img * HW + i];
-  v = float(double(v) * mult);
+  // weighted contexts normalise each image by its own weight sum, as
+  // WSMSGridder::Invert finalizes with 1 / totalWeight (wsmsgridder.cpp:436)
+  const double m = weightSums ? mult / weightSums[img * 4] : mult;
+  v = float(double(v) * m);
   if (correct) {
     const float xv = corrX[x >= W / 2 ? x - W / 2 : W / 2 - x];
     const float yv = corrY[y >= H / 2 ? y - H / 2 : H / 2 - y];
@@ -2275,6 +2318,10 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl) {
   Plan& pl = g->plan;
   pl.valid = false;
+  // imaging weights belong to the planned device path: planWeights sets them
+  // up after this plan, the host protocol grids its staged rows unweighted
+  pl.weighted = false;
+  g->weightSumsValid = false;
   const size_t n = nrow * nchan;
   if (n >= (size_t(1) << 30))
     return setLastError(g, HWS_ERR_UNSUPPORTED,
@@ -2515,6 +2562,9 @@ GridArgs gridArgs(const hws_gridder* g, int pass) {
   a.nchan = pl.nchan;
   a.chunk = pl.chunk;
   a.rot = pl.rot;
+  a.imgw = (g->weights && pl.weighted) ? pl.imgw : nullptr;
+  a.visw = g->dVisWeights;
+  a.visWeightMode = g->visWeightMode;
   return a;
 }
 }  // namespace
@@ -2778,6 +2828,9 @@ PlainArgs plainArgs(const hws_gridder* g, int img, int pass, size_t nrow,
   a.nrow = nrow;
   a.nchan = nchan;
   a.blocked = g->blocked;
+  a.imgw = (g->weights && g->plan.weighted) ? g->plan.imgw : nullptr;
+  a.visw = g->dVisWeights;
+  a.visWeightMode = g->visWeightMode;
   return a;
 }
 }  // namespace
@@ -2862,14 +2915,15 @@ hws_status finalizeImages(hws_gridder* g, double mult, float* dReal,
   const size_t HW = size_t(g->W) * g->H;
   const double norm = 1.0 / (double(g->OS) * double(g->OS));
   const int correct = g->cfg.kernel_mode != HWS_KERNEL_NEAREST_NEIGHBOUR;
+  const double* wsum = (g->weights && g->weightSumsValid) ? g->dWeightSums : nullptr;
   hipLaunchKernelGGL(finalizeKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0, g->stream,
                      g->W, g->H, g->dSum, g->dComp, mult, g->dCorrXf, g->dCorrYf, norm, correct,
-                     dReal);
+                     wsum, dReal);
   HWS_TRY(checkHip(g, hipGetLastError(), "finalize launch"));
   if (dImag && g->anyComplex) {
     hipLaunchKernelGGL(finalizeKernel, dim3(blocksFor(HW), g->nImg), dim3(kThreads), 0, g->stream,
                        g->W, g->H, g->dISum, g->dIComp, mult, g->dCorrXf, g->dCorrYf, norm,
-                       correct, dImag);
+                       correct, wsum, dImag);
     HWS_TRY(checkHip(g, hipGetLastError(), "finalize launch"));
   }
   return HWS_OK;
@@ -2918,6 +2972,150 @@ hws_status imageToLayers(hws_gridder* g, int pass) {
                      g->stream, a, g->dModelRe, g->dModelIm,
                      reinterpret_cast<float2*>(g->dLayers));
   return checkHip(g, hipGetLastError(), "image-to-layers launch");
+}
+
+}  // namespace hws
+
+// ------------------------------------------------------ imaging weights
+namespace hws {
+namespace {
+
+constexpr int kWeightThreads = 256;
+
+// The imaging weight of every (row, channel) sample: GetWeight at
+// (u, v) = uvw / lambda (gridding/msgridderbase.cpp:584-589), stored as the
+// float the reference keeps.
+__global__ void __launch_bounds__(kWeightThreads)
+imagingWeightsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
+                     size_t nchan, WeightGridArgs wg, float* __restrict__ out) {
+  const size_t n = nrow * nchan;
+  for (size_t s = size_t(blockIdx.x) * kWeightThreads + threadIdx.x; s < n;
+       s += size_t(gridDim.x) * kWeightThreads) {
+    const size_t row = s / nchan, ch = s - row * nchan;
+    const double lambda = wl[ch];
+    out[s] = float(imagingWeightAt(wg, uvw[row * 3] / lambda, uvw[row * 3 + 1] / lambda));
+  }
+}
+
+struct WeightSumArgs {
+  LayerMap maps[2];
+  int nImg;
+  int kindOf[kMaxImages];
+  int mode;
+};
+
+// Per-block partial sums per image of the counters readAndWeightVisibilities
+// keeps (gridding/msgridderbase.cpp:640-659), over the samples whose w-layer
+// the image kind grids (the others are deselected, :563-565, and add
+// nothing): [block][image][0 total weight, 1 visibility weight sum, 2 max
+// gridded weight, 3 gridded count].
+__global__ void __launch_bounds__(kWeightThreads)
+weightSumsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
+                 size_t nchan, const float* __restrict__ imgw, const float* __restrict__ visw,
+                 WeightSumArgs a, double* __restrict__ partials) {
+  __shared__ double red[kWeightThreads];
+  double acc[kMaxImages][4];
+#pragma unroll
+  for (int i = 0; i < kMaxImages; ++i) acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0.0;
+  const size_t n = nrow * nchan;
+  const size_t per = (n + gridDim.x - 1) / gridDim.x;
+  const size_t s0 = size_t(blockIdx.x) * per, s1 = s0 + per < n ? s0 + per : n;
+  for (size_t s = s0 + threadIdx.x; s < s1; s += kWeightThreads) {
+    const size_t row = s / nchan, ch = s - row * nchan;
+    const double w = uvw[row * 3 + 2] / wl[ch];
+    const bool sel[2] = {layerOf(a.maps[0], w) >= 0, layerOf(a.maps[1], w) >= 0};
+    for (int i = 0; i < a.nImg; ++i) {
+      if (!sel[a.kindOf[i]]) continue;
+      float vw = visw ? visw[s * size_t(a.nImg) + size_t(i)] : 1.0f;
+      if (a.mode == 1)
+        vw = vw * vw;
+      else if (a.mode == 2)
+        vw = vw != 0.0f ? 1.0f : 0.0f;
+      const double cum = double(vw * imgw[s]);
+      if (cum != 0.0) {
+        acc[i][0] = __dadd_rn(acc[i][0], cum);
+        acc[i][1] = __dadd_rn(acc[i][1], double(vw));
+        acc[i][2] = fmax(acc[i][2], cum);
+        acc[i][3] += 1.0;
+      }
+    }
+  }
+  for (int i = 0; i < a.nImg; ++i)
+    for (int q = 0; q < 4; ++q) {
+      red[threadIdx.x] = acc[i][q];
+      __syncthreads();
+      for (int off = kWeightThreads / 2; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off) {
+          const double o = red[threadIdx.x + off];
+          red[threadIdx.x] = q == 2 ? fmax(red[threadIdx.x], o) : red[threadIdx.x] + o;
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) partials[(size_t(blockIdx.x) * kMaxImages + i) * 4 + q] = red[0];
+      __syncthreads();
+    }
+}
+
+// Sums the block partials in block order (one thread per image and counter).
+__global__ void weightSumsFinalKernel(const double* __restrict__ partials, int nblocks, int nImg,
+                                      double* __restrict__ sums) {
+  const int t = threadIdx.x;
+  if (t >= nImg * 4) return;
+  const int i = t / 4, q = t - i * 4;
+  double r = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    const double v = partials[(size_t(b) * kMaxImages + i) * 4 + q];
+    r = q == 2 ? fmax(r, v) : __dadd_rn(r, v);
+  }
+  sums[i * 4 + q] = r;
+}
+
+}  // namespace
+
+hws_status planWeights(hws_gridder* g) {
+  Plan& pl = g->plan;
+  g->weightSumsValid = false;
+  pl.weighted = false;
+  if (!g->weights || !pl.valid) return HWS_OK;
+  const size_t n = pl.nrow * pl.nchan;
+  if (!g->dWeightSums)
+    HWS_TRY(checkHip(g, hipMalloc(&g->dWeightSums, kMaxImages * 4 * sizeof(double)),
+                     "weight sums"));
+  const int nblocks = 1024;
+  if (g->weightPartialCapacity < size_t(nblocks) * kMaxImages * 4) {
+    if (g->dWeightPartials) (void)hipFree(g->dWeightPartials);
+    g->dWeightPartials = nullptr;
+    g->weightPartialCapacity = 0;
+    HWS_TRY(checkHip(g, hipMalloc(&g->dWeightPartials,
+                                  size_t(nblocks) * kMaxImages * 4 * sizeof(double)),
+                     "weight partial sums"));
+    g->weightPartialCapacity = size_t(nblocks) * kMaxImages * 4;
+  }
+  if (n > 0) {
+    HWS_TRY(ensureAlloc(g, pl.imgw, pl.imgwCapacity, n, "plan imaging weights"));
+    hipLaunchKernelGGL(imagingWeightsKernel, dim3(unsigned(std::min<size_t>(
+                                                 (n + kWeightThreads - 1) / kWeightThreads,
+                                                 size_t(8192)))),
+                       dim3(kWeightThreads), 0, g->stream, pl.uvw, pl.wavelengths, pl.nrow,
+                       pl.nchan, weightGridArgs(g->weights), pl.imgw);
+    HWS_TRY(checkHip(g, hipGetLastError(), "imaging weights launch"));
+  }
+  WeightSumArgs a{};
+  a.maps[0] = layerMap(g, 0);
+  a.maps[1] = layerMap(g, 1);
+  a.nImg = g->nImg;
+  for (int i = 0; i < g->nImg; ++i) a.kindOf[i] = g->isComplex[i] ? 1 : 0;
+  a.mode = g->visWeightMode;
+  hipLaunchKernelGGL(weightSumsKernel, dim3(nblocks), dim3(kWeightThreads), 0, g->stream,
+                     pl.uvw, pl.wavelengths, pl.nrow, pl.nchan, pl.imgw, g->dVisWeights, a,
+                     g->dWeightPartials);
+  HWS_TRY(checkHip(g, hipGetLastError(), "weight sums launch"));
+  hipLaunchKernelGGL(weightSumsFinalKernel, dim3(1), dim3(64), 0, g->stream,
+                     g->dWeightPartials, nblocks, g->nImg, g->dWeightSums);
+  HWS_TRY(checkHip(g, hipGetLastError(), "weight sums launch"));
+  pl.weighted = true;
+  g->weightSumsValid = true;
+  return HWS_OK;
 }
 
 }  // namespace hws

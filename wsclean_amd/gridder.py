@@ -217,6 +217,10 @@ class DeviceGridder(_Context):
 
     def plan(self, uvw_m, wavelengths):
         nrow, nchan = _checks.plan_args(uvw_m, wavelengths, self.device)
+        w = getattr(self, "_weighting", None)
+        if w is not None and w[1] is not None and w[1].numel() != nrow * nchan * self.n_images:
+            raise ValueError(f"the attached vis_weights hold {w[1].numel()} elements, the plan "
+                             f"needs {nrow * nchan * self.n_images}")
         self._uvw, self._wl = uvw_m, wavelengths  # keep alive
         self.nrow, self.nchan = nrow, nchan
         self.planned = False
@@ -249,6 +253,36 @@ class DeviceGridder(_Context):
 
     def synchronize(self):
         self._check(self.L.hws_synchronize(self.h))
+
+    def set_weighting(self, image_weights, vis_weights=None, mode="normal"):
+        """Grid every visibility times visibility weight x imaging weight, as
+        readAndWeightVisibilities weights them (gridding/msgridderbase.cpp:
+        563-589, 640-659), and normalise each inverted image by its own weight
+        sum (gridding/wsmsgridder.cpp:436). image_weights: a finished
+        DeviceImageWeights on this device (None detaches); vis_weights:
+        float32 [nrow, nchan, n_images] (None: 1); mode: "normal", "squared"
+        or "unit". Takes effect from the next plan() (or invert() of the
+        current plan); the sums are in stats()."""
+        import torch
+        from .weights import VISIBILITY_WEIGHTING
+        if image_weights is None:
+            if vis_weights is not None:
+                raise ValueError("visibility weights need imaging weights")
+            self._check(self.L.hws_set_weighting(self.h, None, None, 0))
+            self._weighting = None
+            return
+        if image_weights.device != self.device:
+            raise ValueError("imaging weights live on another device")
+        if mode not in VISIBILITY_WEIGHTING:
+            raise ValueError(f"visibility weighting mode must be one of {list(VISIBILITY_WEIGHTING)}")
+        if vis_weights is not None:
+            _checks.tensor(vis_weights, "vis_weights", torch.float32, device=self.device)
+            if self.planned and vis_weights.numel() != self.nrow * self.nchan * self.n_images:
+                raise ValueError(f"vis_weights has {vis_weights.numel()} elements, expected "
+                                 f"{self.nrow * self.nchan * self.n_images}")
+        self._check(self.L.hws_set_weighting(self.h, image_weights.h, _ptr(vis_weights),
+                                             VISIBILITY_WEIGHTING[mode]))
+        self._weighting = (image_weights, vis_weights)  # keep alive while attached
 
     def capture_invert(self, vis, multiplication_factor, real_out, imag_out=None):
         """A HIP graph of invert() on the planned samples (single pass);

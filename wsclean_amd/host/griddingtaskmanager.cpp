@@ -1,5 +1,6 @@
 #include "griddingtaskmanager.h"
 
+#include "hipdirectftgridder.h"
 #include "hipwsmsgridder.h"
 
 #include <stdexcept>
@@ -33,6 +34,9 @@ GriddingResult GriddingTaskManager::runDirect(GriddingTask&& task, MSGridderBase
   gridder.SetTrimSize(settings_.trimmedImageWidth, settings_.trimmedImageHeight);
   gridder.SetLShift(task.l_shift);
   gridder.SetMShift(task.m_shift);
+  // XY and YX are complex images (griddingtaskmanager.cpp:105-107)
+  gridder.SetPolarization(task.complexPolarization ? aocommon::Polarization::XY
+                                                   : task.polarization);
   gridder.SetIsComplex(task.complexPolarization);
   gridder.SetIsFirstIteration(task.verbose);
   if (task.cache)
@@ -66,10 +70,13 @@ std::unique_ptr<MSGridderBase> GriddingTaskManager::constructGridder(
     const Resources& resources) const {
   switch (settings_.gridderType) {
     case GridderType::HipWStacking:
-      return std::unique_ptr<MSGridderBase>(new HipWSMSGridder(settings_, resources));
+      return std::unique_ptr<MSGridderBase>(new ::HipWSMSGridder(settings_, resources));
+    case GridderType::DirectFT:
+      return std::unique_ptr<MSGridderBase>(new ::HipDirectFTGridder(settings_, resources));
     default:
       throw std::runtime_error(
-          "only the MI355X w-stacking gridder (hip-wstacking) is available in this build");
+          "only the MI355X w-stacking (hip-wstacking) and direct FT (direct-ft) gridders are "
+          "available in this build");
   }
 }
 

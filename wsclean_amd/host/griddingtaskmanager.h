@@ -25,6 +25,7 @@ struct GriddingTask {
   bool imagePSF = false;
   bool subtractModel = false;
   bool complexPolarization = false;  // XY / YX (griddingtaskmanager.cpp:106-107)
+  aocommon::PolarizationEnum polarization = aocommon::Polarization::StokesI;
   bool verbose = false;
   std::unique_ptr<MetaDataCache> cache;
   std::shared_ptr<ImageWeights> imageWeights;

@@ -1,12 +1,14 @@
-// Float image of width x height, row-major, x fastest (the role
-// aocommon::Image plays for the reference gridders).
+// Float image of width x height, row-major, x fastest: the part of
+// aocommon::Image (external/aocommon, an un-vendored submodule of the
+// reference) that the gridders use, under the same name so that a gridder
+// written against the reference's MSGridderBase compiles against this mirror.
 #ifndef WSCLEAN_AMD_HOST_IMAGE_H
 #define WSCLEAN_AMD_HOST_IMAGE_H
 
 #include <cstddef>
 #include <vector>
 
-namespace wsclean_amd {
+namespace aocommon {
 
 class Image {
  public:
@@ -47,6 +49,10 @@ class Image {
   std::vector<float> data_;
 };
 
+}  // namespace aocommon
+
+namespace wsclean_amd {
+using aocommon::Image;
 }  // namespace wsclean_amd
 
 #endif

@@ -44,6 +44,7 @@ struct Settings {
   double nWLayersFactor = 1.0;
   size_t widthForNWCalculation = 0, heightForNWCalculation = 0;
   double wLimit = 0.0;  // percent
+  bool smallInversion = true;  // main/settings.h:267
   WeightMode weightMode = WeightMode::Uniform;
   double briggsRobustness = 0.0;
   VisibilityWeightingMode visibilityWeightingMode = VisibilityWeightingMode::Normal;
