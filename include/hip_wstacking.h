@@ -172,7 +172,10 @@ hws_status hws_pass_layer_range(const hws_gridder* g, size_t pass,
 hws_status hws_start_inversion_pass(hws_gridder* g, size_t pass);
 /* Replaces: AddData (h:288-289, cpp:971-982) for a block of rows, i.e.
  * AddDataSample (cpp:517-597) for every (row, channel) with
- * u = u_m / wavelength etc. Host pointers; vis must be pre-weighted. */
+ * u = u_m / wavelength etc. Host pointers; vis must be pre-weighted.
+ * Each call plans its own block of rows, which replaces a plan made by
+ * hws_plan_device (the *_planned calls then return HWS_ERR_BAD_STATE until
+ * hws_plan_device is called again). */
 hws_status hws_add_data(hws_gridder* g, size_t nrow, size_t nchan,
                         const double* uvw_m, const double* wavelengths,
                         const float* vis);
@@ -195,7 +198,8 @@ hws_status hws_initialize_prediction(hws_gridder* g, const float* real,
 /* Replaces: StartPredictionPass (h:366, cpp:139-157). */
 hws_status hws_start_prediction_pass(hws_gridder* g, size_t pass);
 /* Replaces: SampleData (h:385-386, cpp:984-994) for a block of rows; vis_out
- * [nrow][nchan][n_images]; NaN where this pass predicts nothing. */
+ * [nrow][nchan][n_images]; NaN where this pass predicts nothing. Replaces a
+ * plan made by hws_plan_device, as hws_add_data does. */
 hws_status hws_sample_data(hws_gridder* g, size_t nrow, size_t nchan,
                            const double* uvw_m, const double* wavelengths,
                            float* vis_out);
@@ -220,7 +224,9 @@ hws_status hws_get_kernel(int kernel_mode, size_t oversampling, size_t size,
  * The context must have been prepared with hws_prepare_w_layers. */
 hws_status hws_set_stream(hws_gridder* g, void* hip_stream);
 /* Bin (row, channel) samples by (w-layer, uv tile) for both image kinds; the
- * plan is kept in the context and used by the *_planned calls below. */
+ * plan is kept in the context and used by the *_planned calls below, until
+ * hws_prepare_w_layers, hws_add_data or hws_sample_data replaces it. The
+ * plan reads d_uvw_m and d_wavelengths again at every planned call. */
 hws_status hws_plan_device(hws_gridder* g, size_t nrow, size_t nchan,
                            const double* d_uvw_m, const double* d_wavelengths);
 /* Whole invert over all passes: real/imag [n_images][H][W] device pointers,
@@ -368,8 +374,11 @@ hws_status hws_initialize_prediction_resampled(hws_gridder* g, const float* real
  * executable HIP graph, so a
  * major cycle replays it with one launch. The graph keeps the pointers given
  * here and the context's plan buffers: it is valid until the context is
- * planned again, re-prepared or destroyed. Phase timings (hws_get_stats) are
- * not recorded by graph launches. Captures run on the context's own stream;
+ * planned again (hws_plan_device, hws_add_data, hws_sample_data, the staged
+ * calls), re-prepared, given another weighting (hws_set_weighting) or
+ * destroyed; hws_graph_launch then returns HWS_ERR_BAD_STATE rather than
+ * replay stale pointers. Phase timings (hws_get_stats) are not recorded by
+ * graph launches. Captures run on the context's own stream;
  * hws_graph_launch enqueues a replay on hip_stream (NULL: the null stream). */
 typedef struct hws_graph hws_graph;
 hws_status hws_graph_capture_invert(hws_gridder* g, const float* d_vis,

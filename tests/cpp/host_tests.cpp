@@ -41,6 +41,7 @@ int failures = 0;
   } while (0)
 
 void run(const char* name, const std::function<void()>& fn) {
+  std::printf("-- %s\n", name);
   const int before = failures;
   try {
     fn();
@@ -491,7 +492,7 @@ void gpuTests() {
     std::mt19937 rng(3);
     std::normal_distribution<float> gauss;
     Image model(48, 48);
-    for (size_t i = 0; i != model.Size(); i += 7) model[i] = gauss(rng);
+    for (size_t i = 0; i < model.Size(); i += 7) model[i] = gauss(rng);
     std::unique_ptr<GriddingTaskManager> tm = GriddingTaskManager::Make(s);
     GriddingTask t;
     t.operation = GriddingTask::Predict;
@@ -528,6 +529,8 @@ void gpuTests() {
 
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
+  // line by line, so that a test which dies still leaves the names before it
+  std::setvbuf(stdout, nullptr, _IOLBF, 0);
   if (mode == "cpu" || mode == "all") cpuTests();
   if (mode == "gpu" || mode == "all") gpuTests();
   std::printf("%d failure(s)\n", failures);

@@ -25,6 +25,11 @@ def _case(w, nrow=600, nchan=4, seed=21):
     return uvw, wl, vis, dl, maxw
 
 
+def _rel(a, b):
+    return float(torch.linalg.vector_norm((a - b).double()) /
+                 torch.linalg.vector_norm(b.double()))
+
+
 @pytest.mark.parametrize("precision", ["mixed", "reference"])
 def test_graph_replays_match_eager(precision):
     import wsclean_amd
@@ -91,3 +96,91 @@ def test_graph_capture_rejects_multipass_and_plain():
     with pytest.raises(HwsError, match="batched"):
         p.capture_invert(d_vis, 1.0, img)
     p.close()
+
+
+def test_stale_graphs_and_replaced_plans_are_refused():
+    """A graph bakes in the plan, the layer buffers and the weighting of its
+    context: after any of them changes, or the context is destroyed, a
+    launch is refused instead of replaying stale pointers. A plan made by
+    the host protocol (hws_add_data) replaces the device plan, and the
+    planned calls refuse it until the next plan()."""
+    import ctypes as C
+    import wsclean_amd
+    from wsclean_amd._lib import HwsError
+    from wsclean_amd.weights import DeviceImageWeights
+    w = 64
+    uvw, wl, vis, dl, maxw = _case(w, nrow=80)
+    dev = torch.device("cuda:0")
+    d_uvw, d_wl = torch.from_numpy(uvw).to(dev), torch.from_numpy(wl).to(dev)
+    d_vis = torch.from_numpy(vis[:, :, :1].copy()).to(dev)
+    img = torch.zeros((1, w, w), dtype=torch.float32, device=dev)
+    g = wsclean_amd.DeviceGridder(w, w, dl, dl, 7, 63, precision="mixed")
+    g.prepare_w_layers(8, 0.0, maxw)
+    g.plan(d_uvw, d_wl)
+    ref = torch.zeros_like(img)
+    g.invert(d_vis, 1.0, ref)
+    torch.cuda.synchronize()
+
+    # re-planned: stale; captured again: replays the eager result
+    gr = g.capture_invert(d_vis, 1.0, img)
+    gr.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(img, ref)
+    g.plan(d_uvw, d_wl)
+    with pytest.raises(HwsError, match="capture again"):
+        gr.launch()
+    gr.close()
+    gr = g.capture_invert(d_vis, 1.0, img)
+    img.zero_()
+    gr.launch()
+    torch.cuda.synchronize()
+    # a new plan may hold a bin's records in another order (the binning
+    # scatter places them with atomics), which changes the FP32 sums at
+    # rounding level: equal to the eager image within that
+    assert _rel(img, ref) < 1e-6
+
+    # another weighting: stale
+    iw = DeviceImageWeights("uniform", w, w, dl, dl)
+    iw.grid(d_uvw, d_wl)
+    iw.finish()
+    g.set_weighting(iw)
+    with pytest.raises(HwsError, match="capture again"):
+        gr.launch()
+    gr.close()
+    g.set_weighting(None)
+
+    # re-prepared: the plan is gone and the graph stale
+    gr = g.capture_invert(d_vis, 1.0, img)
+    g.prepare_w_layers(8, 0.0, maxw)
+    with pytest.raises(HwsError, match="capture again"):
+        gr.launch()
+    with pytest.raises(HwsError, match="hws_plan_device has not been called"):
+        g.invert(d_vis, 1.0, img)
+    gr.close()
+
+    # the host protocol replaces the device plan
+    g.plan(d_uvw, d_wl)
+    g._check(g.L.hws_start_inversion_pass(g.h, 0))
+    host_uvw = np.ascontiguousarray(uvw[:10])
+    host_vis = np.ascontiguousarray(vis[:10, :, :1])
+    g._check(g.L.hws_add_data(g.h, 10, len(wl), host_uvw.ctypes.data_as(C.POINTER(C.c_double)),
+                              wl.ctypes.data_as(C.POINTER(C.c_double)),
+                              host_vis.ctypes.data_as(C.POINTER(C.c_float))))
+    g._check(g.L.hws_finish_inversion_pass(g.h))
+    with pytest.raises(HwsError, match="replaced by hws_add_data"):
+        g.invert(d_vis, 1.0, img)
+    with pytest.raises(HwsError, match="replaced by hws_add_data"):
+        g.capture_invert(d_vis, 1.0, img)
+    g.plan(d_uvw, d_wl)
+    img.zero_()
+    g.invert(d_vis, 1.0, img)
+    torch.cuda.synchronize()
+    assert _rel(img, ref) < 1e-6
+
+    # a destroyed context: its graphs refuse to launch
+    gr = g.capture_invert(d_vis, 1.0, img)
+    g.close()
+    with pytest.raises(HwsError, match="destroyed"):
+        gr.launch()
+    gr.close()
+    iw.close()

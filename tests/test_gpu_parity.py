@@ -50,8 +50,8 @@ def lambda_samples(uvw, wl):
 
 
 def oracle_invert(oracle, w, h, dl, dm, uvw_l, vis, nw, minw, maxw, cplx=False, mode="kb",
-                  prec="float", shift=(0.0, 0.0)):
-    g = oracle.RefGridder(w, h, dl, dm, 2, 7, 63, prec)
+                  prec="float", shift=(0.0, 0.0), K=7, OS=63):
+    g = oracle.RefGridder(w, h, dl, dm, 2, K, OS, prec)
     if mode != "kb":
         g.set_grid_mode(mode)
     g.set_is_complex(cplx)
@@ -67,8 +67,8 @@ def oracle_invert(oracle, w, h, dl, dm, uvw_l, vis, nw, minw, maxw, cplx=False, 
 
 
 def oracle_predict(oracle, w, h, dl, dm, uvw_l, real, imag, nw, minw, maxw, cplx=False,
-                   mode="kb", prec="float", shift=(0.0, 0.0)):
-    g = oracle.RefGridder(w, h, dl, dm, 2, 7, 63, prec)
+                   mode="kb", prec="float", shift=(0.0, 0.0), K=7, OS=63):
+    g = oracle.RefGridder(w, h, dl, dm, 2, K, OS, prec)
     if mode != "kb":
         g.set_grid_mode(mode)
     g.set_is_complex(cplx)
@@ -81,9 +81,9 @@ def oracle_predict(oracle, w, h, dl, dm, uvw_l, real, imag, nw, minw, maxw, cplx
 
 
 def gpu_invert(hws, w, h, dl, dm, uvw, wl, vis, nw, minw, maxw, precision, complex_images=(),
-               mode="kb", layers=False, max_mem=0.0, **kw):
+               mode="kb", layers=False, max_mem=0.0, K=7, OS=63, **kw):
     nimg = vis.shape[2]
-    g = hws.WStackingGridder(w, h, dl, dm, 7, 63, kernel_mode=mode, precision=precision,
+    g = hws.WStackingGridder(w, h, dl, dm, K, OS, kernel_mode=mode, precision=precision,
                              n_images=nimg, complex_images=complex_images, **kw)
     npass = g.prepare_w_layers(nw, max_mem, minw, maxw)
     lay = None
@@ -166,6 +166,40 @@ def test_complex_and_real_images_together(hws, oracle):
     v1 = oracle_predict(oracle, w, w, dl, dl, uvw_l, mre[1], mim[1], 8, 0.0, maxw, cplx=True)
     assert rel(v[:, :, 0].reshape(-1), v0) < TOL_IMG["reference"]
     assert rel(v[:, :, 1].reshape(-1), v1) < TOL_IMG["reference"]
+
+
+def test_four_images_beyond_lds_split_per_launch(hws, oracle):
+    # 56-pixel uv tiles with a 9-tap kernel give 64 x 64 degridding subgrids
+    # (32 KiB per image); beside the 1023-times oversampled kernel table
+    # (36.8 KB) four images need 172 KB of LDS, more than a workgroup's
+    # 160 KiB, so the degridder runs them as three and one (and the gridder
+    # splits its private tiles the same way)
+    w, K, OS = 224, 9, 1023
+    uvw, wl, vis, dl = case(12, w, nimg=4)
+    uvw_l = lambda_samples(uvw, wl)
+    maxw = np.abs(uvw_l[:, 2]).max()
+    re, _, _, _ = gpu_invert(hws, w, w, dl, dl, uvw, wl, vis, 8, 0.0, maxw, "reference",
+                             K=K, OS=OS, tile_size=56)
+    rng = np.random.default_rng(121)
+    models = rng.standard_normal((4, w, w)).astype(np.float32)
+    g = hws.WStackingGridder(w, w, dl, dl, K, OS, precision="reference", n_images=4,
+                             tile_size=56)
+    g.prepare_w_layers(8, 0.0, 0.0, maxw)
+    g.initialize_prediction(models)
+    g.start_prediction_pass(0)
+    v = g.sample_data(uvw, wl)
+    for k in range(4):
+        img_o, _ = oracle_invert(oracle, w, w, dl, dl, uvw_l, vis[:, :, k].reshape(-1), 8, 0.0,
+                                 maxw, K=K, OS=OS)
+        ei = rel(re[k], img_o)
+        v_o = oracle_predict(oracle, w, w, dl, dl, uvw_l, models[k], None, 8, 0.0, maxw, K=K,
+                             OS=OS)
+        vk = v[:, :, k].reshape(-1)
+        assert np.array_equal(np.isnan(vk), np.isnan(v_o))
+        ok = ~np.isnan(v_o)
+        ep = rel(vk[ok], v_o[ok])
+        print(f"K={K} OS={OS} T=56 image {k}: invert rel {ei:.2e}, predict rel {ep:.2e}")
+        assert ei < TOL_IMG["reference"] and ep < TOL_IMG["reference"]
 
 
 def test_nearest_neighbour_mode(hws, oracle):

@@ -22,6 +22,8 @@ transform through hws_direct_*_device.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from . import synth
@@ -153,3 +155,106 @@ def report(device="cuda:0", precisions=("mixed", "reference", "plain"), n_sample
             if prec in precisions:
                 out[f"{prec}_os{os_}"] = rec
     return out
+
+
+def _image_error(W, H, dl, dm, uvw, one, vis, n, n_w, w_lo, w_hi, dft_img, cutout, os_,
+                 precision, kernel_size, device, n_images=1):
+    import torch
+    from .gridder import DeviceGridder
+    g = DeviceGridder(W, H, dl, dm, kernel_size, os_, precision=precision, n_images=n_images)
+    g.prepare_w_layers(n_w, w_lo, w_hi)
+    img = torch.zeros((n_images, H, W), dtype=torch.float32, device=torch.device(device))
+    g.plan(uvw, one)
+    g.invert(vis, 1.0 / n, img)
+    g.synchronize()
+    g.close()
+    lo_y, lo_x = (H - cutout) // 2, (W - cutout) // 2
+    im = img.cpu().numpy()[0, lo_y:lo_y + cutout, lo_x:lo_x + cutout].astype(np.float64)
+    return _rel(im, dft_img)[0]
+
+
+def sweep(device="cuda:0", n_ws=(32, 48, 64, 96, 128, 192, 256), oversamplings=(63, 1023),
+          precision="mixed", n_samples=100_000, seed=303):
+    """Image and visibility errors of the precision protocol (the report's
+    workload) against the FP64 direct transform, for every w-layer count in
+    n_ws and every kernel oversampling: where the w-stacking approximation
+    (each sample on its nearest layer) stops dominating the error, and what
+    the extra layers cost. The reference's suggestion for this workload is
+    reported beside it (suggested_w_layers in autotune)."""
+    import torch
+    from .autotune import suggested_w_layers
+    from .gridder import DeviceGridder
+
+    wk = precision_workload(n_samples=n_samples, seed=seed)
+    W, dl, cut = wk["width"], wk["dl"], wk["cutout"]
+    n = wk["uvw_lambda"].shape[0]
+    dev = torch.device(device)
+    uvw = torch.from_numpy(np.ascontiguousarray(wk["uvw_lambda"])).to(dev)
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+    vis = torch.from_numpy(wk["vis"]).to(dev).reshape(n, 1, 1)
+    model = torch.from_numpy(wk["model"]).to(dev)
+    aw = np.abs(wk["uvw_lambda"][:, 2])
+    w_lo, w_hi = float(aw.min()), float(aw.max())
+    dft_img, dft_vis = direct_reference(wk, device)
+    rows = []
+    for os_ in oversamplings:
+        for n_w in n_ws:
+            g = DeviceGridder(W, W, dl, dl, 7, os_, precision=precision, n_images=1)
+            g.prepare_w_layers(n_w, w_lo, w_hi)
+            img = torch.zeros((1, W, W), dtype=torch.float32, device=dev)
+            g.plan(uvw, one)
+            g.invert(vis, 1.0 / n, img)
+            pred = torch.empty_like(vis)
+            g.predict(model.reshape(1, W, W), None, pred)
+            g.synchronize()
+            g.close()
+            i_rms, v_rms = errors(img.cpu().numpy()[0], pred.cpu().numpy(), dft_img, dft_vis,
+                                  cut)
+            rows.append({"n_w_layers": n_w, "oversampling": os_, "image_rms_rel": i_rms,
+                         "vis_rms_rel": v_rms})
+    return {"precision": precision,
+            "suggested_w_layers": suggested_w_layers(W, W, dl, dl, w_lo, w_hi),
+            "w_range": (w_lo, w_hi), "rows": rows}
+
+
+def w_layers_for_accuracy(width, height, dl, dm, uvw_m, wavelengths, vis, bound, *,
+                          n_w_start, n_w_max=1024, growth=1.5, oversampling=63,
+                          kernel_size=7, precision="mixed", n_samples=20_000, cutout=None,
+                          seed=0):
+    """The smallest w-layer count from n_w_start up (growing by `growth`) at
+    which the dirty image of a random subset of the caller's samples is
+    within `bound` (RMS over the central cutout, relative to the peak) of
+    the FP64 direct transform of the same samples. uvw_m [nrow, 3],
+    wavelengths [nchan] and vis [nrow, nchan, n_images] are device tensors
+    as DeviceGridder takes them; the first image's visibilities are used.
+    Returns (n_w or None, [(n_w, error), ...])."""
+    import torch
+    from .direct import direct_invert
+    dev = uvw_m.device
+    nrow, nchan = uvw_m.shape[0], wavelengths.shape[0]
+    rng = np.random.default_rng(seed)
+    total = nrow * nchan
+    pick = np.sort(rng.choice(total, size=min(n_samples, total), replace=False))
+    idx = torch.from_numpy(pick).to(dev)
+    rows, chans = idx // nchan, idx % nchan
+    uvw = (uvw_m[rows] / wavelengths[chans][:, None]).contiguous()
+    v = vis.reshape(nrow * nchan, -1)[idx, 0].contiguous().reshape(-1, 1, 1)
+    n = uvw.shape[0]
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+    cut = cutout or min(width, height) // 2
+    dft = direct_invert(uvw, one, v.reshape(n, 1), cut, cut, dl, dm,
+                        multiplication_factor=1.0 / n)
+    torch.cuda.synchronize()
+    dft_img = dft.cpu().numpy()
+    aw = torch.abs(uvw[:, 2])
+    w_lo, w_hi = float(aw.min()), float(aw.max())
+    tried = []
+    n_w = max(1, int(n_w_start))
+    while n_w <= n_w_max:
+        e = _image_error(width, height, dl, dm, uvw, one, v, n, n_w, w_lo, w_hi, dft_img, cut,
+                         oversampling, precision, kernel_size, str(dev))
+        tried.append((n_w, e))
+        if e <= bound:
+            return n_w, tried
+        n_w = max(n_w + 1, int(math.ceil(n_w * growth)))
+    return None, tried

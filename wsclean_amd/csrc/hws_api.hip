@@ -30,6 +30,12 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// hws_add_data / hws_sample_data plan each block of rows they are given,
+// which replaces a plan made by hws_plan_device.
+constexpr const char* kReplacedPlan =
+    "the device plan was replaced by hws_add_data or hws_sample_data: call hws_plan_device "
+    "again";
+
 // ---- 1-D kernels: the closed forms of gridding/wstackinggridder.cpp:425-515
 double bessel0(double x, double precision) {
   double d = 0.0, ds = 1.0, sum = 1.0;
@@ -204,7 +210,7 @@ int layersInPass(const hws_gridder* g, int pass) {
 // fromModel (forward, with sparse): the row pass computes the layers from the
 // model images (imageToLayers is not run).
 hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false,
-                         bool fromModel = false) {
+                         bool fromModel = false, bool rowsOnly = false) {
   const int n = layersInPass(g, pass);
   const size_t HW = size_t(g->W) * g->H;
   for (int img = 0; img < g->nImg; ++img) {
@@ -227,10 +233,12 @@ hws_status fftPassLayers(hws_gridder* g, int pass, int sign, bool sparse = false
       const KindPlan& kp = pl.kinds[g->isComplex[img] ? 1 : 0];
       sp.occ = sign > 0 ? kp.occ : kp.need;
       sp.rowLive = kp.rowLive;
+      sp.colNeed = (sign < 0 && fromModel && !g->noFusedProject) ? kp.colNeed : nullptr;
       sp.layer0 = passLayerStart(g, pass);
       sp.tilesY = pl.tilesY;
       sp.tshift = 0;
       while ((1 << sp.tshift) < pl.tileSize) ++sp.tshift;
+      sp.rowsOnly = rowsOnly;
     }
     HWS_TRY(fft2d(g, g->dLayers + size_t(img) * g->maxLayersPerPass * HW, n, sign,
                   sparse ? &sp : nullptr, fromModel ? &ms : nullptr));
@@ -397,6 +405,8 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   {
     const char* e = std::getenv("HWS_NO_MFMA");
     g->noMfma = e && e[0] == '1';
+    const char* f = std::getenv("HWS_NO_FUSED_PROJECT");
+    g->noFusedProject = f && f[0] == '1';
   }
   // uv tile edge of the binned forms: divides W and H, a multiple of 4, at
   // least K - 1 (a footprint then spans at most two tiles per axis) and with
@@ -429,6 +439,7 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
 
 void hws_destroy(hws_gridder* g) {
   if (!g) return;
+  g->generation->alive = false;  // graphs of this context refuse to launch
   if (g->cfg.device >= 0) (void)hipSetDevice(g->cfg.device);
   if (g->ownStream) (void)hipStreamSynchronize(g->ownStream);
   if (g->userStream) (void)hipStreamSynchronize(g->stream);
@@ -474,6 +485,7 @@ void hws_destroy(hws_gridder* g) {
     freeDev(kp.occ);
     freeDev(kp.need);
     freeDev(kp.rowLive);
+    freeDev(kp.colNeed);
     freeDev(kp.partials);
     freeDev(kp.skipped);
     freeDev(kp.keys);
@@ -515,6 +527,10 @@ hws_status hws_prepare_w_layers(hws_gridder* g, size_t n_w_layers,
   // gridding/wstackinggridder.cpp:70-75
   if (g->minW == g->maxW) g->maxW += 1.0;
   g->nW = int(n_w_layers);
+  // the plan's bins follow the layer map, and the layer buffers may move:
+  // plan again, capture again
+  g->plan.valid = false;
+  ++g->generation->value;
   const size_t HW = size_t(g->W) * g->H;
   const size_t perLayer = layersBytes(g, 1);
   double budget = max_mem;
@@ -763,6 +779,7 @@ hws_status hws_plan_device(hws_gridder* g, size_t nrow, size_t nchan,
   HWS_TRY(record(g, 0));
   if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32) {
     Plan& pl = g->plan;
+    ++g->generation->value;
     pl.nrow = nrow;
     pl.nchan = nchan;
     pl.uvw = d_uvw_m;
@@ -771,6 +788,7 @@ hws_status hws_plan_device(hws_gridder* g, size_t nrow, size_t nchan,
   } else {
     HWS_TRY(buildPlan(g, nrow, nchan, d_uvw_m, d_wavelengths));
   }
+  g->plan.device = true;
   HWS_TRY(planWeights(g));
   HWS_TRY(record(g, 1));
   g->stats = hws_stats{};
@@ -783,6 +801,7 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
                                      double mult, float* d_real, float* d_imag) {
   if (!g) return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "null context");
   if (!g->plan.valid) return fail(g, HWS_ERR_BAD_STATE, "hws_plan_device has not been called");
+  if (!g->plan.device) return fail(g, HWS_ERR_BAD_STATE, kReplacedPlan);
   if (!d_real || (g->plan.nrow && !d_vis)) return fail(g, HWS_ERR_INVALID_ARGUMENT, "null data pointer");
   (void)hipSetDevice(g->cfg.device);
   Plan& pl = g->plan;
@@ -809,10 +828,15 @@ hws_status hws_invert_planned_device(hws_gridder* g, const float* d_vis,
       HWS_TRY(gridBinned(g, d_vis, pass, false, sparseInvert(g)));
     }
     HWS_TRY(record(g, 3));
-    HWS_TRY(fftPassLayers(g, pass, +1,
-                          g->cfg.precision != HWS_PRECISION_PLAIN_FP32 && sparseInvert(g)));
+    const bool sparse = !plainForm && sparseInvert(g);
+    // the columns pass and the projection in one kernel where it applies
+    const bool fused = sparse && colsProjectSupported(g, layersInPass(g, pass));
+    HWS_TRY(fftPassLayers(g, pass, +1, sparse, false, fused));
     HWS_TRY(record(g, 4));
-    HWS_TRY(projectPass(g, pass, !plainForm && pass == 0));
+    if (fused)
+      HWS_TRY(projectFusedPass(g, pass, pass == 0));
+    else
+      HWS_TRY(projectPass(g, pass, !plainForm && pass == 0));
     HWS_TRY(record(g, 5));
     if (g->nPasses > 1) {
       float a = 0, b = 0, c = 0;
@@ -841,6 +865,7 @@ hws_status hws_predict_planned_device(hws_gridder* g, const float* d_real,
                                       const float* d_imag, float* d_vis_out) {
   if (!g) return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "null context");
   if (!g->plan.valid) return fail(g, HWS_ERR_BAD_STATE, "hws_plan_device has not been called");
+  if (!g->plan.device) return fail(g, HWS_ERR_BAD_STATE, kReplacedPlan);
   if (!d_real || (g->plan.nrow && !d_vis_out)) return fail(g, HWS_ERR_INVALID_ARGUMENT, "null data pointer");
   if (g->anyComplex && !d_imag)
     return fail(g, HWS_ERR_INVALID_ARGUMENT, "Missing imaginary in complex prediction");
@@ -981,6 +1006,7 @@ hws_status hws_set_weighting(hws_gridder* g, const hws_weights* weights,
   g->weights = weights;
   g->dVisWeights = weights ? d_vis_weights : nullptr;
   g->visWeightMode = weights ? visibility_weighting_mode : 0;
+  ++g->generation->value;  // graphs captured with the old weighting are stale
   // the imaging weights of the samples and the weight sums are computed by
   // the next hws_plan_device, or by the next invert of the current plan
   g->plan.weighted = false;
@@ -994,6 +1020,8 @@ struct hws_graph {
   hipGraphExec_t exec = nullptr;
   int device = 0;
   std::vector<hipEvent_t> events;  // fork/join events of a batch capture
+  // each captured context's plan generation at the capture
+  std::vector<std::pair<std::shared_ptr<PlanGeneration>, uint64_t>> generations;
 };
 
 }  // extern "C"
@@ -1018,6 +1046,7 @@ hws_status graphChecks(hws_gridder* g, hws_graph** out) {
   if (g->cfg.precision == HWS_PRECISION_PLAIN_FP32)
     return fail(g, HWS_ERR_UNSUPPORTED, "graph capture is for the batched precisions");
   if (g->capturing) return fail(g, HWS_ERR_BAD_STATE, "a capture is already in progress");
+  if (!g->plan.device) return fail(g, HWS_ERR_BAD_STATE, kReplacedPlan);
   return HWS_OK;
 }
 
@@ -1103,6 +1132,8 @@ hws_status captureBatch(hws_gridder* const* gs, size_t n, Op op, hws_graph** out
   gr->exec = exec;
   gr->device = g->cfg.device;
   gr->events = std::move(events);
+  for (size_t i = 0; i < n; ++i)
+    gr->generations.emplace_back(gs[i]->generation, gs[i]->generation->value.load());
   *out = gr;
   return HWS_OK;
 }
@@ -1189,6 +1220,14 @@ hws_status hws_graph_capture_predict_batch(hws_gridder* const* gs, size_t n,
 
 hws_status hws_graph_launch(hws_graph* gr, void* hip_stream) {
   if (!gr) return fail(nullptr, HWS_ERR_INVALID_ARGUMENT, "null graph");
+  for (const auto& gen : gr->generations) {
+    if (!gen.first->alive)
+      return fail(nullptr, HWS_ERR_BAD_STATE, "a context of the graph has been destroyed");
+    if (gen.first->value != gen.second)
+      return fail(nullptr, HWS_ERR_BAD_STATE,
+                  "the plan, w-layers or weighting of a context changed after the capture: "
+                  "capture again");
+  }
   (void)hipSetDevice(gr->device);
   return checkHip(nullptr, hipGraphLaunch(gr->exec, static_cast<hipStream_t>(hip_stream)),
                   "graph launch");

@@ -22,8 +22,13 @@ namespace hws {
 
 namespace {
 
+// The butterflies and twiddle products fuse only where written (fmaf), so a
+// transform computes the same bits in every kernel that inlines it; left to
+// itself, the compiler contracts a product into the following sum in some
+// inlining contexts and not in others.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+#pragma clang fp contract(off)
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) {
   return make_float2(a.x + b.x, a.y + b.y);
@@ -43,6 +48,7 @@ struct Butterfly;
 template <int SIGN>
 struct Butterfly<SIGN, 2> {
   __device__ __forceinline__ static void run(float2* v) {
+#pragma clang fp contract(off)
     float2 a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -52,6 +58,7 @@ struct Butterfly<SIGN, 2> {
 template <int SIGN>
 struct Butterfly<SIGN, 4> {
   __device__ __forceinline__ static void run(float2* v) {
+#pragma clang fp contract(off)
     float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
     float2 b0 = cadd(v[1], v[3]), b1 = mulI<SIGN>(csub(v[1], v[3]));
     v[0] = cadd(a0, b0);
@@ -64,6 +71,7 @@ struct Butterfly<SIGN, 4> {
 template <int SIGN>
 struct Butterfly<SIGN, 8> {
   __device__ __forceinline__ static void run(float2* v) {
+#pragma clang fp contract(off)
     // radix-2 over even/odd, then radix-4 on each half
     float2 e[4] = {v[0], v[2], v[4], v[6]};
     float2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -86,6 +94,7 @@ struct Butterfly<SIGN, 8> {
 template <int SIGN, int R>
 struct Butterfly {
   __device__ __forceinline__ static void run(float2* v) {
+#pragma clang fp contract(off)
     float2 res[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -243,6 +252,7 @@ __device__ __forceinline__ int padIdx(int e) { return e + (e >> 6); }
 template <int SIGN>
 struct Butterfly<SIGN, 16> {
   __device__ __forceinline__ static void run(float2* v) {
+#pragma clang fp contract(off)
     // 4 x 4: DFT4 over stride-4 inputs, twiddle w16^(n1 k2), DFT4 again
     float2 a[4][4];
 #pragma unroll
@@ -298,25 +308,14 @@ __device__ __forceinline__ float2 modelCell(const ModelSource& ms, double w, int
   if (yDest >= H) yDest -= H;
   int xDest = (W - x) + W / 2;
   if (xDest >= W) xDest -= W;
-  const double m = ((double)yDest - (H / 2)) * ms.dm + ms.mShift;
-  const double l = ((W / 2) - (double)xDest) * ms.dl + ms.lShift;
+  const double m = pixelM(yDest, H, ms.dm, ms.mShift);
+  const double l = pixelL(xDest, W, ms.dl, ms.lShift);
   double np = nPrimeSeries(l, m);
   if (REF) np = double(float(np));
   const size_t src = size_t(yDest) * W + xDest;
   const float re = ms.re[src];
   const float imv = ms.cplx ? -ms.im[src] : 0.0f;
-  if (REF) {
-    double sn, cs;
-    sincos(2.0 * 3.14159265358979323846 * w * np, &sn, &cs);
-    if (ms.cplx) return make_float2(float(re * cs + imv * sn), float(imv * cs - re * sn));
-    return make_float2(float(re * cs), float(-re * sn));
-  }
-  const double phi = w * np;
-  const float f = float(phi - rint(phi));
-  float sn, cs;
-  turnSinCos(f, sn, cs);
-  if (ms.cplx) return make_float2(re * cs + imv * sn, imv * cs - re * sn);
-  return make_float2(re * cs, -re * sn);
+  return modelTerm<REF>(re, imv, ms.cplx != 0, w, np);
 }
 
 // Layers per workgroup of the model-fed rows pass: the workgroup reads the
@@ -554,6 +553,327 @@ fftColsBlockedKernel(float2* __restrict__ data, int W, const float2* __restrict_
   if ((nb & 7) == 0) cb = (cb & 7) * (nb >> 3) + (cb >> 3);
   float2* g = data + size_t(blockIdx.y) * size_t(W) * size_t(N) + size_t(cb) * 16;
   blockedFft<SIGN, N, false>(lds, g, W, tw, threadIdx.x);
+}
+
+// ---- the backward columns pass fused with the w-projection
+// One Stockham stage of radix R over NF interleaved transforms of length N
+// (NF N / 16 threads, 16 values per thread), as blockedStage but with
+// registers for memory: the first stage takes its inputs from io
+// (io[q R + r] = element b + r N / R of transform f, for bt = tid + q T,
+// f = bt mod NF, b = bt / NF) and the last stage leaves its outputs there
+// (io[q R + r] = element ((b / NS) NS R + b mod NS) + r NS of transform f).
+template <int SIGN, int R, int N, int NS, int NF>
+__device__ __forceinline__ void regStage(float2* lds, const float2* __restrict__ tw, int tid,
+                                         const float2* in, float2* out) {
+  constexpr int B = N / R;
+  constexpr int T = NF * N / 16;
+  constexpr int PER = 16 / R;
+  constexpr int TWSTEP = N / (NS * R);
+  constexpr bool FIRST = NS == 1;
+  constexpr bool LAST = NS * R == N;
+  float2 v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int bt = tid + q * T;
+    const int f = bt % NF, b = bt / NF;
+    const int jm = b % NS;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float2 x = FIRST ? in[q * R + r] : lds[padIdx((b + r * B) * NF + f)];
+      if (!FIRST && r != 0) x = cmul(x, tw[jm * r * TWSTEP]);
+      v[q][r] = x;
+    }
+  }
+  // every read of this stage precedes its writes (and, in the first stage,
+  // the previous transform's last reads precede this one's writes: that
+  // stage synchronised between its reads and its outputs)
+  if (!FIRST) __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int bt = tid + q * T;
+    const int f = bt % NF, b = bt / NF;
+    const int jm = b % NS;
+    Butterfly<SIGN, R>::run(v[q]);
+    const int base = (b / NS) * NS * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (LAST)
+        out[q * R + r] = v[q][r];
+      else
+        lds[padIdx((base + r * NS) * NF + f)] = v[q][r];
+    }
+  }
+  if (!LAST) __syncthreads();
+}
+
+// The transform of in (consumed by the first stage) into out; afterFirst()
+// runs once the first stage has read in (in may then be overwritten, e.g. by
+// the loads of the next transform's inputs).
+template <int SIGN, int N, int NF, int NS = 1, class F>
+__device__ __forceinline__ void regFft(float2* lds, const float2* __restrict__ tw, int tid,
+                                       const float2* in, float2* out, F&& afterFirst) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = (REM % 16 == 0) ? 16 : REM;
+    static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix");
+    regStage<SIGN, R, N, NS, NF>(lds, tw, tid, in, out);
+    if constexpr (NS == 1) afterFirst();
+    regFft<SIGN, N, NF, NS * R>(lds, tw, tid, in, out, afterFirst);
+  }
+}
+
+// The last stage of regFft over length N: its radix and stride.
+template <int N>
+struct LastStage {
+  static constexpr int p16() {
+    int p = 1;
+    while ((N / p) % 16 == 0) p *= 16;
+    return p;
+  }
+  static constexpr int R = N / p16() == 1 ? 16 : N / p16();
+  static constexpr int NS = N / R;
+};
+
+constexpr int kProjectColumns = 2;  // columns per workgroup of the fused pass
+
+// Buffer-resource word 3 for raw (untyped) buffer loads on CDNA.
+constexpr int kRawBufferFlags = 0x00020000;
+// An offset past every buffer: such a buffer load returns zero without a
+// memory access.
+constexpr uint32_t kOffsetOutOfRange = 0xFFFFFFF0u;
+
+// Backward columns pass (after fftRowsBlockedSparseKernel) fused with
+// projectOnImageAndCorrect (gridding/wstackinggridder.cpp:874-907) for every
+// layer of the pass: a workgroup owns two columns of one image and transforms
+// them layer after layer, in the pass's layer order; each transformed cell is
+// rotated by its layer's w term and added (Kahan) to its pixel's sums, which
+// stay in registers. So the transformed layers are never written, and the
+// accumulators are read (unless the pass is the first) and written once per
+// pass. Cells of tile rows without samples read as zero, as in
+// fftColsBlockedRowSparseKernel. The next layer's inputs are loaded while the
+// current layer is transformed. Every term is computed and added as
+// projectKernel computes and adds it, so the sums are the same bits.
+// grid (W / 2, images of the kind); dynamic LDS: the transform buffer and
+// the row bits of the pass's layers.
+template <int N, bool REF, bool CPLX>
+__global__ void __launch_bounds__(kProjectColumns * N / 16)
+fftColsProjectKernel(ColsProjectArgs a, const float2* __restrict__ layers,
+                     const float2* __restrict__ tw, float* __restrict__ sum,
+                     float* __restrict__ comp, float* __restrict__ isum,
+                     float* __restrict__ icomp) {
+  constexpr int NF = kProjectColumns;
+  constexpr int T = NF * N / 16;
+  constexpr int B0 = N / 16;  // the first stage is radix 16: one butterfly per thread
+  constexpr int RL = LastStage<N>::R, NSL = LastStage<N>::NS, PERL = 16 / RL;
+  extern __shared__ float2 fusedLds[];
+  float2* lds = fusedLds;
+  // n' of this thread's 16 output cells: [k][thread] (registers would not
+  // hold them beside the transform, the prefetch and the sums)
+  double* npl = reinterpret_cast<double*>(fusedLds + NF * N + (NF * N) / 64);
+  uint32_t* bits = reinterpret_cast<uint32_t*>(npl + 16 * T);
+  const int tid = int(threadIdx.x);
+  const int W = a.W;
+  const size_t HW = size_t(W) * N;
+  const int img = a.img[blockIdx.y];
+  const bool fresh = a.fresh != 0;
+  // column pairs sharing 128-byte segments run on one XCD, one after the other
+  const int nb = int(gridDim.x);
+  int cp = int(blockIdx.x);
+  if ((nb & 7) == 0) cp = (cp & 7) * (nb >> 3) + (cp >> 3);
+  const int x0 = cp * NF;
+  // the row bits of the pass's layers: bit li * tilesY + ty
+  const int nl = a.nLayersInPass;
+  const size_t nbits = size_t(nl) * a.tilesY;
+  const size_t bit0 = size_t(a.passStart) * a.tilesY;
+  const size_t w0 = bit0 >> 5;
+  const int sh = int(bit0 & 31);
+  for (size_t wd = size_t(tid); wd < (nbits + 31) / 32; wd += T) {
+    uint32_t v = a.rowLive[w0 + wd] >> sh;
+    if (sh != 0 && w0 + wd + 1 < a.rowLiveWords) v |= a.rowLive[w0 + wd + 1] << (32 - sh);
+    bits[wd] = v;
+  }
+  // this thread's output pixels: n' and the accumulators
+  float s[16], c[16], is[CPLX ? 16 : 1], ic[CPLX ? 16 : 1];
+#pragma unroll
+  for (int q = 0; q < PERL; ++q) {
+    const int bt = tid + q * T;
+    const int f = bt % NF, b = bt / NF;
+    const int base = (b / NSL) * NSL * RL + b % NSL;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int k = q * RL + r;
+      const int y = base + r * NSL, x = x0 + f;
+      int ySrc = (N - y) + N / 2;
+      if (ySrc >= N) ySrc -= N;
+      int xSrc = x + W / 2;
+      if (xSrc >= W) xSrc -= W;
+      const double m = pixelM(ySrc, N, a.dm, a.mShift);
+      const double l = pixelL(xSrc, W, a.dl, a.lShift);
+      double np = nPrime(l, m);
+      if (REF) np = double(float(np));
+      npl[k * T + tid] = np;
+      const size_t o = size_t(img) * HW + size_t(ySrc) * W + xSrc;
+      // (read also in a fresh pass, where the values are not used: no
+      // branch around the loads)
+      s[k] = fresh ? 0.0f : sum[o];
+      c[k] = fresh ? 0.0f : comp[o];
+      if (CPLX) {
+        is[k] = fresh ? 0.0f : isum[o];
+        ic[k] = fresh ? 0.0f : icomp[o];
+      }
+    }
+  }
+  __syncthreads();  // the row bits
+  const int f0 = tid % NF, b0 = tid / NF;
+  const float2* col = layers + size_t(img) * a.nLayersPass * HW + size_t(x0) * 4;
+  float2 in[16], out[16];
+  // cells of tile rows without samples were never written: their loads
+  // take an offset out of the buffer's range and return zero (no branch
+  // around the loads, and 32-bit offsets against a per-layer resource)
+  const int colBytes = int((HW - size_t(x0) * 4) * sizeof(float2));
+  auto load = [&](int li, float2* dst) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2*>(col + size_t(li) * HW), 0, colBytes, kRawBufferFlags);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int pos = b0 + r * B0;
+      const int bit = li * a.tilesY + (pos >> a.tshift);
+      const uint32_t off = ((bits[bit >> 5] >> (bit & 31)) & 1u)
+                               ? uint32_t(blockedAddr<false>(pos, f0, W) * sizeof(float2))
+                               : kOffsetOutOfRange;
+      const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+      // (the elements by value: a bit cast of a vector element would read
+      // element 0 for both with this compiler)
+      const unsigned lo = v[0], hi = v[1];
+      dst[r] = make_float2(__uint_as_float(lo), __uint_as_float(hi));
+    }
+  };
+  load(0, in);
+#pragma unroll 1
+  for (int li = 0; li < nl; ++li) {
+    // the next layer's inputs are loaded once the first stage has read this one's
+    regFft<+1, N, NF>(lds, tw, tid, in, out, [&] {
+      if (li + 1 < nl) load(li + 1, in);
+    });
+    const double w = layerToW(a.map, a.passStart + li);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float2 t = projectTerm<REF>(out[k], w, npl[k * T + tid]);
+      kahanAdd(s[k], c[k], t.x);
+      if (CPLX) kahanAdd(is[k], ic[k], t.y);
+      // one cell's term at a time: interleaving the sixteen would hold
+      // their temporaries at once
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  const bool zeroImag = !CPLX && fresh && isum != nullptr;
+#pragma unroll
+  for (int q = 0; q < PERL; ++q) {
+    const int bt = tid + q * T;
+    const int f = bt % NF, b = bt / NF;
+    const int base = (b / NSL) * NSL * RL + b % NSL;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int k = q * RL + r;
+      const int y = base + r * NSL, x = x0 + f;
+      int ySrc = (N - y) + N / 2;
+      if (ySrc >= N) ySrc -= N;
+      int xSrc = x + W / 2;
+      if (xSrc >= W) xSrc -= W;
+      const size_t o = size_t(img) * HW + size_t(ySrc) * W + xSrc;
+      sum[o] = s[k];
+      comp[o] = c[k];
+      if (CPLX) {
+        isum[o] = is[k];
+        icomp[o] = ic[k];
+      } else if (zeroImag) {
+        // a fresh pass also zeroes the imaginary accumulators of real images
+        isum[o] = 0.0f;
+        icomp[o] = 0.0f;
+      }
+    }
+  }
+}
+
+// Forward rows pass computed from the model for every layer of the batch
+// (copyImageToLayerAndInverseCorrect, gridding/wstackinggridder.cpp:934-969,
+// then the row transforms): a workgroup owns two rows of one image, reads
+// their model values and computes their n' once, and then, layer after
+// layer, forms each cell's w-phased input (modelTerm, as modelCell forms it),
+// transforms the two rows and writes the tile columns the columns pass reads
+// (colNeed: one bit per (layer, tile column); the columns pass skips the
+// others). The cells' model values and n' stay in registers across the
+// layers, so a layer costs the phase, the transform and the stores.
+// grid (H / 2): row pair rp = rows 2 rp, 2 rp + 1, pairs of a four-row block
+// run on one XCD one after the other (their stores fill the same lines).
+template <int N, bool REF, bool CPLX>
+__global__ void __launch_bounds__(kProjectColumns * N / 16)
+fftRowsModelLayersKernel(float2* __restrict__ data, int H, int nbatch,
+                         const float2* __restrict__ tw, ModelSource ms,
+                         const uint32_t* __restrict__ colNeed, int tshift) {
+  constexpr int NF = kProjectColumns;
+  constexpr int T = NF * N / 16;
+  constexpr int B0 = N / 16;
+  constexpr int RL = LastStage<N>::R, NSL = LastStage<N>::NS, PERL = 16 / RL;
+  __shared__ float2 lds[NF * N + (NF * N) / 64];
+  const int tid = int(threadIdx.x);
+  const int W = N;
+  const size_t HW = size_t(W) * H;
+  const int nb = int(gridDim.x);
+  int rp = int(blockIdx.x);
+  if ((nb & 7) == 0) rp = (rp & 7) * (nb >> 3) + (rp >> 3);
+  const int y0 = rp * NF;
+  const int tilesX = W >> tshift;
+  // this thread's first-stage cells: x = b0 + r N / 16 of row y0 + f0
+  const int f0 = tid % NF, b0 = tid / NF;
+  float re[16], imv[CPLX ? 16 : 1];
+  double np[16];
+  {
+    const int y = y0 + f0;
+    int yDest = y + H / 2;
+    if (yDest >= H) yDest -= H;
+    const double m = pixelM(yDest, H, ms.dm, ms.mShift);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int x = b0 + r * B0;
+      int xDest = (W - x) + W / 2;
+      if (xDest >= W) xDest -= W;
+      const double l = pixelL(xDest, W, ms.dl, ms.lShift);
+      np[r] = nPrimeSeries(l, m);
+      if (REF) np[r] = double(float(np[r]));
+      const size_t src = size_t(yDest) * W + xDest;
+      re[r] = ms.re[src];
+      if (CPLX) imv[r] = -ms.im[src];
+    }
+  }
+  // the rows' cells in the blocked layout: row y0 + f of a layer at
+  // rowBase + layer offset + x * 4 + f
+  const size_t rowBase = size_t(y0 >> 2) * 4 * W + size_t(y0 & 3);
+  float2 in[16], out[16];
+#pragma unroll 1
+  for (int b = 0; b < nbatch; ++b) {
+    const int layer = ms.layer0 + b;
+    const double w = layerToW(ms.map, layer);
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      in[r] = modelTerm<REF>(re[r], CPLX ? imv[r] : 0.0f, CPLX, w, np[r]);
+    regFft<-1, N, NF>(lds, tw, tid, in, out, [] {});
+    float2* g = data + size_t(b) * HW + rowBase;
+    const size_t cbit = size_t(layer) * tilesX;
+#pragma unroll
+    for (int q = 0; q < PERL; ++q) {
+      const int bt = tid + q * T;
+      const int f = bt % NF, bb = bt / NF;
+      const int base = (bb / NSL) * NSL * RL + bb % NSL;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int x = base + r * NSL;
+        const size_t bit = cbit + size_t(x >> tshift);
+        if ((colNeed[bit >> 5] >> (bit & 31)) & 1u)
+          streamStore(&g[size_t(x) * 4 + f], out[q * RL + r]);
+      }
+    }
+  }
 }
 
 // rows of length N, NFFT rows per block (the last block may hold fewer)
@@ -883,6 +1203,67 @@ bool blockedLayoutSupported(int W, int H) {
   return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && W <= 2044 && H <= 2044;
 }
 
+namespace {
+size_t colsProjectLds(int H, int nLayersInPass, int tilesY) {
+  const size_t words = (size_t(nLayersInPass) * tilesY + 31) / 32;
+  const size_t n = size_t(kProjectColumns) * H;
+  // transform buffer, n' of every output cell, row bits
+  return (n + n / 64) * sizeof(float2) + n * sizeof(double) + words * sizeof(uint32_t);
+}
+}  // namespace
+
+bool colsProjectSupported(const hws_gridder* g, int nLayersInPass) {
+  if (!g->blocked || !fastBlockedSize(g->W) || !fastBlockedSize(g->H) || g->noFusedProject)
+    return false;
+  const int T = g->plan.tileSize;
+  if (T < 4 || (T & (T - 1)) != 0) return false;
+  int maxLds = 65536;
+  (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, g->cfg.device);
+  return colsProjectLds(g->H, nLayersInPass, g->H / T) <= size_t(maxLds);
+}
+
+hws_status launchColsProject(hws_gridder* g, const ColsProjectArgs& a, bool cplx,
+                             const float2* layers, float* sum, float* comp, float* isum,
+                             float* icomp) {
+  if (a.nImg == 0 || a.nLayersInPass == 0) return HWS_OK;
+  const bool ref = g->cfg.precision == HWS_PRECISION_REFERENCE;
+  const dim3 grid(unsigned(a.W / kProjectColumns), unsigned(a.nImg));
+  const size_t lds = colsProjectLds(a.H, a.nLayersInPass, a.tilesY);
+  const float2* tw = g->colPlan.twBackward;
+  bool ok = false;
+  switch (a.H) {
+#define HWS_COLS_PROJECT(NN)                                                                    \
+  case NN:                                                                                      \
+    if (ref && cplx)                                                                            \
+      hipLaunchKernelGGL((fftColsProjectKernel<NN, true, true>), grid,                          \
+                         dim3(kProjectColumns * NN / 16), lds, g->stream, a, layers, tw, sum,   \
+                         comp, isum, icomp);                                                    \
+    else if (ref)                                                                               \
+      hipLaunchKernelGGL((fftColsProjectKernel<NN, true, false>), grid,                         \
+                         dim3(kProjectColumns * NN / 16), lds, g->stream, a, layers, tw, sum,   \
+                         comp, isum, icomp);                                                    \
+    else if (cplx)                                                                              \
+      hipLaunchKernelGGL((fftColsProjectKernel<NN, false, true>), grid,                         \
+                         dim3(kProjectColumns * NN / 16), lds, g->stream, a, layers, tw, sum,   \
+                         comp, isum, icomp);                                                    \
+    else                                                                                        \
+      hipLaunchKernelGGL((fftColsProjectKernel<NN, false, false>), grid,                        \
+                         dim3(kProjectColumns * NN / 16), lds, g->stream, a, layers, tw, sum,   \
+                         comp, isum, icomp);                                                    \
+    ok = true;                                                                                  \
+    break;
+    HWS_COLS_PROJECT(256)
+    HWS_COLS_PROJECT(512)
+    HWS_COLS_PROJECT(1024)
+    HWS_COLS_PROJECT(2048)
+    HWS_COLS_PROJECT(4096)
+#undef HWS_COLS_PROJECT
+    default: break;
+  }
+  if (!ok) return setLastError(g, HWS_ERR_UNSUPPORTED, "fused projection: height not supported");
+  return checkHip(g, hipGetLastError(), "fused columns and projection launch");
+}
+
 hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSparse* sparse,
                  const ModelSource* model) {
   if (nbatch <= 0) return HWS_OK;
@@ -968,8 +1349,8 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
 #undef HWS_SPARSE_CASE
         default: break;
       }
-      bool cols = false;
-      switch (H) {
+      bool cols = sparse->rowsOnly;
+      if (!cols) switch (H) {
 #define HWS_ROWSPARSE_COLS(NN)                                                                 \
   case NN:                                                                                     \
     hipLaunchKernelGGL((fftColsBlockedRowSparseKernel<+1, NN>), dim3(W / 4, nbatch),            \
@@ -987,7 +1368,42 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
       }
       ok = ok && cols;
     } else if (sign < 0 && sparse) {
-      if (model) {
+      if (model && sparse->colNeed) {
+        // every layer per workgroup, the model and n' read and computed once
+        ok = false;
+        const bool ref = model->precision == HWS_PRECISION_REFERENCE;
+        const bool cplx = model->cplx != 0;
+        const dim3 mgrid(H / kProjectColumns);
+        switch (W) {
+#define HWS_MODEL_LAYERS(NN)                                                                   \
+  case NN:                                                                                     \
+    if (ref && cplx)                                                                           \
+      hipLaunchKernelGGL((fftRowsModelLayersKernel<NN, true, true>), mgrid,                    \
+                         dim3(kProjectColumns * NN / 16), 0, g->stream, data, H, nbatch,        \
+                         twRow, *model, sparse->colNeed, sparse->tshift);                       \
+    else if (ref)                                                                              \
+      hipLaunchKernelGGL((fftRowsModelLayersKernel<NN, true, false>), mgrid,                   \
+                         dim3(kProjectColumns * NN / 16), 0, g->stream, data, H, nbatch,        \
+                         twRow, *model, sparse->colNeed, sparse->tshift);                       \
+    else if (cplx)                                                                             \
+      hipLaunchKernelGGL((fftRowsModelLayersKernel<NN, false, true>), mgrid,                   \
+                         dim3(kProjectColumns * NN / 16), 0, g->stream, data, H, nbatch,        \
+                         twRow, *model, sparse->colNeed, sparse->tshift);                       \
+    else                                                                                       \
+      hipLaunchKernelGGL((fftRowsModelLayersKernel<NN, false, false>), mgrid,                  \
+                         dim3(kProjectColumns * NN / 16), 0, g->stream, data, H, nbatch,        \
+                         twRow, *model, sparse->colNeed, sparse->tshift);                       \
+    ok = true;                                                                                 \
+    break;
+          HWS_MODEL_LAYERS(256)
+          HWS_MODEL_LAYERS(512)
+          HWS_MODEL_LAYERS(1024)
+          HWS_MODEL_LAYERS(2048)
+          HWS_MODEL_LAYERS(4096)
+#undef HWS_MODEL_LAYERS
+          default: break;
+        }
+      } else if (model) {
         ok = false;
         const int ml = model->precision == HWS_PRECISION_REFERENCE ? modelLayers<true>()
                                                                    : modelLayers<false>();

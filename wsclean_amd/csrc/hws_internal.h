@@ -6,8 +6,10 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -51,9 +53,31 @@ struct LayerMap {
   int isComplex;
 };
 
+// The FP64 pixel-geometry helpers below round every product and sum on its
+// own (no contraction into an FMA) or fuse explicitly, so that each kernel
+// that evaluates them gets the same bits wherever the compiler inlines them.
+
+// l and m of image pixel (x, y) (gridding/wstackinggridder.cpp:862-864,
+// 922-924): l = (W/2 - x) dl + l_shift, m = (y - H/2) dm + m_shift.
+__host__ __device__ inline double pixelL(int x, int W, double dl, double lShift) {
+#pragma clang fp contract(off)
+  return ((W / 2) - (double)x) * dl + lShift;
+}
+__host__ __device__ inline double pixelM(int y, int H, double dm, double mShift) {
+#pragma clang fp contract(off)
+  return ((double)y - (H / 2)) * dm + mShift;
+}
+
+// (l, m) inside the unit circle: l^2 + m^2 < 1.
+__host__ __device__ inline bool nPrimeInside(double l, double m) {
+#pragma clang fp contract(off)
+  return l * l + m * m < 1.0;
+}
+
 // n' - 1 at (l, m) (0 outside the unit circle), as the reference's sqrt
 // lookup tables hold it (gridding/wstackinggridder.cpp:851-872, 909-932).
 __host__ __device__ inline double nPrime(double l, double m) {
+#pragma clang fp contract(off)
   return (l * l + m * m < 1.0) ? sqrt(1.0 - l * l - m * m) - 1.0 : 0.0;
 }
 
@@ -63,15 +87,16 @@ __host__ __device__ inline double nPrime(double l, double m) {
 // Six FMAs instead of a correctly rounded FP64 square root, for the
 // per-layer evaluation of the forward transform's input.
 __host__ __device__ inline double nPrimeSeries(double l, double m) {
+#pragma clang fp contract(off)
   const double x = l * l + m * m;
   if (!(x < 0.01)) return nPrime(l, m);
   // -x/2 - x^2/8 - x^3/16 - 5x^4/128 - 7x^5/256 - 21x^6/1024
   double p = -21.0 / 1024.0;
-  p = p * x - 7.0 / 256.0;
-  p = p * x - 5.0 / 128.0;
-  p = p * x - 1.0 / 16.0;
-  p = p * x - 1.0 / 8.0;
-  p = p * x - 1.0 / 2.0;
+  p = fma(p, x, -7.0 / 256.0);
+  p = fma(p, x, -5.0 / 128.0);
+  p = fma(p, x, -1.0 / 16.0);
+  p = fma(p, x, -1.0 / 8.0);
+  p = fma(p, x, -1.0 / 2.0);
   return p * x;
 }
 
@@ -83,6 +108,7 @@ __host__ __device__ inline double nPrimeSeries(double l, double m) {
 // 1e-7 of the exact values; about half the instructions of sincospif, whose
 // general range reduction this argument does not need.
 __device__ __forceinline__ void turnSinCos(float f, float& s, float& c) {
+#pragma clang fp contract(off)
   const float t = 4.0f * f;
   const float k = rintf(t);
   const float r = t - k;
@@ -105,6 +131,59 @@ __device__ __forceinline__ void turnSinCos(float f, float& s, float& c) {
   s = (q & 2) ? -s0 : s0;
   c = ((q + 1) & 2) ? -c0 : c0;
 }
+
+__device__ __forceinline__ void kahanAdd(float& sum, float& comp, float term) {
+  const float y = term - comp;
+  const float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
+
+// One layer's term of projectOnImageAndCorrect (gridding/wstackinggridder.cpp:
+// 874-907): the transformed layer value F at a pixel with n' = np, times
+// exp(-2 pi i w n'). REF: FP64 sincos; otherwise the FP64-reduced fraction of
+// a turn and turnSinCos. The products are fused explicitly and nothing else
+// is (phi - rint(phi) would otherwise become an FMA in some kernels and not
+// in others), so every kernel that projects gets the same bits.
+template <bool REF>
+__device__ __forceinline__ float2 projectTerm(float2 F, double w, double np) {
+#pragma clang fp contract(off)
+  if (REF) {
+    double sn, cs;
+    sincos(-2.0 * 3.14159265358979323846 * w * np, &sn, &cs);
+    return make_float2(float(fma(double(F.x), cs, -(double(F.y) * sn))),
+                       float(fma(double(F.x), sn, double(F.y) * cs)));
+  }
+  const double phi = w * np;
+  const float f = float(phi - rint(phi));
+  float sf, cf;
+  turnSinCos(f, sf, cf);
+  // exp(-2 pi i phi): sin = -sf, cos = cf
+  return make_float2(fmaf(F.x, cf, F.y * sf), fmaf(-F.x, sf, F.y * cf));
+}
+
+// One layer cell of copyImageToLayerAndInverseCorrect (gridding/
+// wstackinggridder.cpp:934-969): the model value (re, imv = -imaginary part
+// for a complex image, else 0) at a pixel with n' = np, times
+// exp(+2 pi i w n'). Precisions and explicit fusing as in projectTerm.
+template <bool REF>
+__device__ __forceinline__ float2 modelTerm(float re, float imv, bool cplx, double w, double np) {
+#pragma clang fp contract(off)
+  if (REF) {
+    double sn, cs;
+    sincos(2.0 * 3.14159265358979323846 * w * np, &sn, &cs);
+    if (cplx)
+      return make_float2(float(fma(double(re), cs, double(imv) * sn)),
+                         float(fma(double(imv), cs, -(double(re) * sn))));
+    return make_float2(float(double(re) * cs), float(-(double(re) * sn)));
+  }
+  const double phi = w * np;
+  const float f = float(phi - rint(phi));
+  float sn, cs;
+  turnSinCos(f, sn, cs);
+  if (cplx) return make_float2(fmaf(re, cs, imv * sn), fmaf(imv, cs, -(re * sn)));
+  return make_float2(re * cs, -(re * sn));
+}
 #endif
 
 // The imaging-weight grid of an ImageWeights (structures/imageweights.h:
@@ -118,16 +197,19 @@ struct WeightGridArgs {
 #ifdef __HIPCC__
 // Cell of (u, v) (wavelengths) in the weight grid, or -1 outside it:
 // uvToXY + isWithinLimits (structures/imageweights.h:73-94). The products and
-// the sum are rounded one by one in the reference's order (no contraction
-// into an FMA), so a sample on a cell boundary lands in the reference's cell.
+// the sum are rounded one by one in the reference's order, so a sample on a
+// cell boundary lands in the reference's cell. The pragma keeps the compiler
+// from contracting them into an FMA (HIP contracts by default, and __dmul_rn /
+// __dadd_rn are plain operators in this toolchain's headers, which does not
+// stop it).
 __device__ inline long long weightCell(const WeightGridArgs& w, double u, double v) {
+#pragma clang fp contract(off)
   if (v < 0.0) {
     u = -u;
     v = -v;
   }
-  const double xf = __dadd_rn(__dmul_rn(__dmul_rn(u, double(w.width)), w.dl),
-                              double(w.width / 2));
-  const double yf = __dmul_rn(__dmul_rn(v, double(w.height)), w.dm);
+  const double xf = u * double(w.width) * w.dl + double(w.width / 2);
+  const double yf = v * double(w.height) * w.dm;
   const double fx = floor(xf), fy = floor(yf);
   // bounded before the integer conversion (the reference converts unchecked)
   if (!(fx >= 0.0) || !(fx < double(w.width)) || !(fy < double(w.height / 2)) || !(fy >= 0.0))
@@ -144,6 +226,7 @@ __device__ inline double imagingWeightAt(const WeightGridArgs& w, double u, doub
 #endif
 
 __host__ __device__ inline double layerToW(const LayerMap& m, int layer) {
+#pragma clang fp contract(off)
   if (m.nW == 1) return 0.0;
   if (m.isComplex) return layer * (m.maxW + m.maxW) / (m.nW - 1) - m.maxW;
   return layer * (m.maxW - m.minW) / (m.nW - 1) + m.minW;
@@ -185,6 +268,9 @@ struct KindPlan {
   // bit per (layer, tile row): some tile of the row holds records
   uint32_t* rowLive = nullptr;
   size_t rowLiveCapacity = 0;
+  // one bit per (layer, tile column) holding a tile the degridding reads
+  uint32_t* colNeed = nullptr;
+  size_t colNeedCapacity = 0;
   // samples of this kind that no record grids (off the grid or off every
   // layer): the predict writes NaN for them [nSkipped]
   uint32_t* skipped = nullptr;
@@ -198,6 +284,9 @@ struct KindPlan {
 
 struct Plan {
   bool valid = false;
+  // made by hws_plan_device (the caller's device rows, which the planned
+  // calls and graphs read) rather than by the host protocol's staged rows
+  bool device = false;
   size_t nrow = 0, nchan = 0;
   const double* uvw = nullptr;         // device
   const double* wavelengths = nullptr;  // device
@@ -238,6 +327,15 @@ struct Plan {
   bool weighted = false;  // imgw holds this plan's samples
 };
 
+// Version of what a captured graph of a context bakes in: the plan and its
+// rows, the w-layer buffers and the imaging weights. Bumped whenever one of
+// them changes; `alive` drops when the context is destroyed. Shared between
+// the context and its graphs, so a graph outliving its context can tell.
+struct PlanGeneration {
+  std::atomic<uint64_t> value{0};
+  std::atomic<bool> alive{true};
+};
+
 }  // namespace hws
 
 // An ImageWeights on one device (structures/imageweights.{h,cpp}): the
@@ -259,6 +357,7 @@ struct hws_weights {
 
 struct hws_gridder {
   hws_config cfg{};
+  std::shared_ptr<hws::PlanGeneration> generation = std::make_shared<hws::PlanGeneration>();
   int W = 0, H = 0, K = 0, OS = 0;
   int nImg = 0;
   bool isComplex[hws::kMaxImages] = {};
@@ -301,6 +400,9 @@ struct hws_gridder {
   // grid with the VALU cell kernel at T = 16 instead of the matrix-core one
   // (HWS_NO_MFMA=1 in the environment; for comparisons)
   bool noMfma = false;
+  // run the backward columns pass and the projection apart, and the forward
+  // rows pass four layers per workgroup (the forms the fused kernels replace)
+  bool noFusedProject = false;
 
   // binning plan
   hws::Plan plan;
@@ -391,6 +493,9 @@ hws_status finalizeImages(hws_gridder* g, double mult, float* dReal,
 hws_status initPrediction(hws_gridder* g, const float* dReal,
                           const float* dImag);
 hws_status imageToLayers(hws_gridder* g, int pass);
+// The projection of a pass fused with its backward columns pass (after the
+// sparse rows pass): both image kinds.
+hws_status projectFusedPass(hws_gridder* g, int pass, bool fresh);
 
 // FFT (hws_fft.hip)
 hws_status makeFftPlan(hws_gridder* g, int n, FftPlan& plan);
@@ -407,7 +512,33 @@ struct FftSparse {
   int layer0;           // layer of the first transform of the batch
   int tilesY;
   int tshift;           // log2 of the tile edge
+  bool rowsOnly = false;  // backward: the columns pass runs fused with the projection
+  // forward from a model: one bit per (layer, tile column) holding a needed
+  // tile (the row pass then writes those columns only), or null
+  const uint32_t* colNeed = nullptr;
 };
+
+// The columns pass of a pass's backward transforms fused with the projection
+// of its layers onto the image accumulators (fftColsProjectKernel): the
+// images of one kind, after their sparse backward rows pass.
+struct ColsProjectArgs {
+  int W, H;
+  double dl, dm, lShift, mShift;
+  LayerMap map;
+  int passStart, nLayersInPass, nLayersPass;
+  int fresh;  // the accumulators start from zero (not read)
+  const uint32_t* rowLive;  // one bit per (layer, tile row) holding samples
+  size_t rowLiveWords;
+  int tilesY, tshift;
+  int nImg;
+  int img[kMaxImages];
+};
+// Whether the fused columns pass and projection covers a transform of
+// height H with this many layers in a pass.
+bool colsProjectSupported(const hws_gridder* g, int nLayersInPass);
+hws_status launchColsProject(hws_gridder* g, const ColsProjectArgs& a, bool cplx,
+                             const float2* layers, float* sum, float* comp, float* isum,
+                             float* icomp);
 // The model image a forward transform's row pass reads instead of the
 // layers: each layer cell is computed on the fly as imageToLayersKernel
 // writes it (copyImageToLayerAndInverseCorrect, wstackinggridder.cpp:934-969).

@@ -592,11 +592,13 @@ popcountKernel(const uint32_t* __restrict__ bits, size_t words,
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, (unsigned long long)v);
 }
 
-// Counts the (layer, tile column) pairs with a set bit in some tile row
-// (thread per pair): the column blocks a sparse forward column pass reads.
+// Column bits: bit (layer * tilesX + tx) is set when some tile row of that
+// tile column has a set bit (thread per bit, packed by wave ballots): the
+// column blocks a sparse forward columns pass reads, and the tile columns the
+// forward rows pass writes.
 __global__ void __launch_bounds__(kThreads)
 liveColumnsKernel(const uint32_t* __restrict__ bits, size_t ncols, int tilesX, int tilesY,
-                  unsigned long long* __restrict__ out) {
+                  uint32_t* __restrict__ colBits) {
   const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool live = false;
   if (c < ncols) {
@@ -607,7 +609,10 @@ liveColumnsKernel(const uint32_t* __restrict__ bits, size_t ncols, int tilesX, i
     }
   }
   const unsigned long long m = __ballot(live);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+  const size_t w0 = (c & ~size_t(63)) >> 5;  // first word of this wave's 64 bits
+  const int lane = threadIdx.x & 63;
+  if (lane == 0 && w0 < (ncols + 31) / 32) colBits[w0] = uint32_t(m);
+  if (lane == 32 && w0 + 1 < (ncols + 31) / 32) colBits[w0 + 1] = uint32_t(m >> 32);
 }
 
 // Need bits: tile (tx, ty) of a layer is read by the degridding subgrids of
@@ -1902,14 +1907,6 @@ struct ImgArgs {
   int fresh;  // the accumulators start from zero (not read)
 };
 
-// n' = sqrt(1 - l^2 - m^2) - 1 inside the horizon, else 0
-// (gridding/wstackinggridder.cpp:865-868, 925-928).
-__device__ __forceinline__ void kahanAdd(float& sum, float& comp, float term) {
-  const float y = term - comp;
-  const float t = sum + y;
-  comp = (t - sum) - y;
-  sum = t;
-}
 
 // Invert: every layer of the pass onto the image, FFT-ordered pixel p.
 // projectOnImageAndCorrect (gridding/wstackinggridder.cpp:874-907) for all
@@ -1929,8 +1926,8 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
   if (ySrc >= a.H) ySrc -= a.H;
   int xSrc = x + a.W / 2;
   if (xSrc >= a.W) xSrc -= a.W;
-  const double m = ((double)ySrc - (a.H / 2)) * a.dm + a.mShift;
-  const double l = ((a.W / 2) - (double)xSrc) * a.dl + a.lShift;
+  const double m = pixelM(ySrc, a.H, a.dm, a.mShift);
+  const double l = pixelL(xSrc, a.W, a.dl, a.lShift);
   double np = nPrime(l, m);
   if (a.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
   const size_t o = size_t(img) * HW + size_t(ySrc) * a.W + xSrc;
@@ -1951,23 +1948,10 @@ projectKernel(ImgArgs a, const float2* __restrict__ layers,
     const f2 fv = __builtin_nontemporal_load(reinterpret_cast<const f2*>(src + size_t(li) * HW));
     const float2 F = make_float2(fv.x, fv.y);
     const double w = layerToW(map, a.passStart + li);
-    float re, im;
-    if (a.precision == HWS_PRECISION_REFERENCE) {
-      double sn, cs;
-      sincos(-2.0 * kPi * w * np, &sn, &cs);
-      re = float(F.x * cs - F.y * sn);
-      im = float(F.x * sn + F.y * cs);
-    } else {
-      const double phi = w * np;
-      const float f = float(phi - rint(phi));
-      float sf, cf;
-      turnSinCos(f, sf, cf);
-      // exp(-2 pi i phi): sin = -sf, cos = cf
-      re = F.x * cf + F.y * sf;
-      im = -F.x * sf + F.y * cf;
-    }
-    kahanAdd(s, c, re);
-    if (cplx) kahanAdd(is, ic, im);
+    const float2 t = a.precision == HWS_PRECISION_REFERENCE ? projectTerm<true>(F, w, np)
+                                                            : projectTerm<false>(F, w, np);
+    kahanAdd(s, c, t.x);
+    if (cplx) kahanAdd(is, ic, t.y);
   }
   sum[o] = s;
   comp[o] = c;
@@ -2021,8 +2005,8 @@ __global__ void nTableKernel(int W, int H, int blocked, double dl, double dm,
     xs = (W - x) + W / 2;
     if (xs >= W) xs -= W;
   }
-  const double m = ((double)ys - (H / 2)) * dm + ms;
-  const double l = ((W / 2) - (double)xs) * dl + ls;
+  const double m = pixelM(ys, H, dm, ms);
+  const double l = pixelL(xs, W, dl, ls);
   out[p] = float(nPrime(l, m));
 }
 
@@ -2064,10 +2048,10 @@ initPredictionKernel(int W, int H, double dl, double dm, double ls, double ms,
   if (i >= HW) return;
   const size_t img = blockIdx.y;
   const int x = int(i % W), y = int(i / W);
-  const double m = ((double)y - (H / 2)) * dm + ms;
-  const double l = ((W / 2) - (double)x) * dl + ls;
+  const double m = pixelM(y, H, dm, ms);
+  const double l = pixelL(x, W, dl, ls);
   float v = model[img * HW + i];
-  v = (isfinite(v) && l * l + m * m < 1.0) ? in[img * HW + i] : 0.0f;
+  v = (isfinite(v) && nPrimeInside(l, m)) ? in[img * HW + i] : 0.0f;
   if (correct) {
     const float xv = corrX[x >= W / 2 ? x - W / 2 : W / 2 - x];
     const float yv = corrY[y >= H / 2 ? y - H / 2 : H / 2 - y];
@@ -2093,8 +2077,8 @@ imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
   if (yDest >= a.H) yDest -= a.H;
   int xDest = (a.W - x) + a.W / 2;
   if (xDest >= a.W) xDest -= a.W;
-  const double m = ((double)yDest - (a.H / 2)) * a.dm + a.mShift;
-  const double l = ((a.W / 2) - (double)xDest) * a.dl + a.lShift;
+  const double m = pixelM(yDest, a.H, a.dm, a.mShift);
+  const double l = pixelL(xDest, a.W, a.dl, a.lShift);
   double np = nPrimeSeries(l, m);  // as the forward transform's fused row pass
   if (a.precision == HWS_PRECISION_REFERENCE) np = double(float(np));
   const size_t src = size_t(img) * HW + size_t(yDest) * a.W + xDest;
@@ -2104,25 +2088,9 @@ imageToLayersKernel(ImgArgs a, const float* __restrict__ modelRe,
   float2* dst = layers + size_t(img) * a.nLayersPass * HW + p;
   for (int li = 0; li < a.nLayersInPass; ++li) {
     const double w = layerToW(map, a.passStart + li);
-    float2 out;
-    if (a.precision == HWS_PRECISION_REFERENCE) {
-      double sn, cs;
-      sincos(2.0 * kPi * w * np, &sn, &cs);
-      if (cplx)
-        out = make_float2(float(re * cs + imv * sn), float(imv * cs - re * sn));
-      else
-        out = make_float2(float(re * cs), float(-re * sn));
-    } else {
-      const double phi = w * np;
-      const float f = float(phi - rint(phi));
-      float sn, cs;
-      turnSinCos(f, sn, cs);
-      if (cplx)
-        out = make_float2(re * cs + imv * sn, imv * cs - re * sn);
-      else
-        out = make_float2(re * cs, -re * sn);
-    }
-    dst[size_t(li) * HW] = out;
+    dst[size_t(li) * HW] = a.precision == HWS_PRECISION_REFERENCE
+                               ? modelTerm<true>(re, imv, cplx != 0, w, np)
+                               : modelTerm<false>(re, imv, cplx != 0, w, np);
   }
 }
 
@@ -2318,6 +2286,8 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
                      const double* dUvw, const double* dWl) {
   Plan& pl = g->plan;
   pl.valid = false;
+  pl.device = false;  // hws_plan_device marks its own plans
+  ++g->generation->value;
   // imaging weights belong to the planned device path: planWeights sets them
   // up after this plan, the host protocol grids its staged rows unweighted
   pl.weighted = false;
@@ -2470,9 +2440,12 @@ hws_status buildPlan(hws_gridder* g, size_t nrow, size_t nchan,
     hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(rowWords)), dim3(kThreads), 0, g->stream,
                        kp.rowLive, rowWords, pl.counters + 14 + kind);
     const size_t ncolsBits = pl.nbins / size_t(pl.tilesY);
+    const size_t colWords = (ncolsBits + 31) / 32;
+    HWS_TRY(ensureAlloc(g, kp.colNeed, kp.colNeedCapacity, colWords, "plan column bits"));
     hipLaunchKernelGGL(liveColumnsKernel, dim3(blocksFor(ncolsBits)), dim3(kThreads), 0,
-                       g->stream, kp.need, ncolsBits, pl.tilesX, pl.tilesY,
-                       pl.counters + 16 + kind);
+                       g->stream, kp.need, ncolsBits, pl.tilesX, pl.tilesY, kp.colNeed);
+    hipLaunchKernelGGL(popcountKernel, dim3(blocksFor(colWords)), dim3(kThreads), 0, g->stream,
+                       kp.colNeed, colWords, pl.counters + 16 + kind);
     HWS_TRY(checkHip(g, hipGetLastError(), "tile count launch"));
     hipLaunchKernelGGL(itemBinsKernel, dim3(blocksFor(pl.nbins)), dim3(kThreads), 0, g->stream,
                        kp.itemStart, pl.nbins, kp.itemBin);
@@ -2767,15 +2740,25 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
   // (measured: 8 per CU, 13.2 ms per step at the 4096^2 benchmark; 5, 12.2)
   const int blocks = persistentBlocks(g, 5);
   const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
+  const size_t ldsMax = maxLdsBytes(g);
+  float2* out = reinterpret_cast<float2*>(dVisOut);
   for (int kind = 0; kind < 2; ++kind) {
     const KindPlan& kp = pl.kinds[kind];
     if (!kp.active) continue;
-    a.imgs.n = kp.nImg;
-    for (int i = 0; i < kMaxImages; ++i) a.imgs.idx[i] = kp.img[i];
-    const size_t lds = gridLdsBytes(g, kp.nImg, true);
-    float2* out = reinterpret_cast<float2*>(dVisOut);
-    switch (kp.nImg) {
-#define HWS_DEGRID_CASE(N)                                                                  \
+    // images per launch: as many subgrids as fit in LDS beside the kernel
+    // table (each launch writes the samples of its own images only)
+    int per = kp.nImg;
+    while (per > 1 && gridLdsBytes(g, per, true) > ldsMax) --per;
+    if (gridLdsBytes(g, per, true) > ldsMax)
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "uv subgrid and kernel table exceed LDS");
+    for (int i0 = 0; i0 < kp.nImg; i0 += per) {
+      const int n = std::min(per, kp.nImg - i0);
+      a.imgs.n = n;
+      for (int i = 0; i < kMaxImages; ++i)
+        a.imgs.idx[i] = (i0 + i < kp.nImg) ? kp.img[i0 + i] : 0;
+      const size_t lds = gridLdsBytes(g, n, true);
+      switch (n) {
+#define HWS_DEGRID_CASE(N)                                                                \
   case N:                                                                                   \
     if (g->K == 7)                                                                          \
       hipLaunchKernelGGL((degridTilesKernel<N, 7>), dim3(blocks), dim3(kThreads), lds,        \
@@ -2788,13 +2771,14 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
                          g->dKernel, khalf, pl.uvw, pl.wavelengths,                         \
                          reinterpret_cast<const float2*>(g->dLayers), out);                 \
     break;
-      HWS_DEGRID_CASE(1)
-      HWS_DEGRID_CASE(2)
-      HWS_DEGRID_CASE(3)
-      HWS_DEGRID_CASE(4)
+        HWS_DEGRID_CASE(1)
+        HWS_DEGRID_CASE(2)
+        HWS_DEGRID_CASE(3)
+        HWS_DEGRID_CASE(4)
 #undef HWS_DEGRID_CASE
+      }
+      HWS_TRY(checkHip(g, hipGetLastError(), "degrid launch"));
     }
-    HWS_TRY(checkHip(g, hipGetLastError(), "degrid launch"));
   }
   return HWS_OK;
 }
@@ -2908,6 +2892,39 @@ hws_status projectPass(hws_gridder* g, int pass, bool fresh) {
                      reinterpret_cast<const float2*>(g->dLayers), g->dSum, g->dComp, g->dISum,
                      g->dIComp);
   return checkHip(g, hipGetLastError(), "project launch");
+}
+
+hws_status projectFusedPass(hws_gridder* g, int pass, bool fresh) {
+  const Plan& pl = g->plan;
+  if (!pl.valid) return setLastError(g, HWS_ERR_BAD_STATE, "no binning plan");
+  int tshift = 0;
+  while ((1 << tshift) < pl.tileSize) ++tshift;
+  const float2* layers = reinterpret_cast<const float2*>(g->dLayers);
+  for (int kind = 0; kind < 2; ++kind) {
+    const KindPlan& kp = pl.kinds[kind];
+    if (!kp.active) continue;
+    ColsProjectArgs a{};
+    a.W = g->W;
+    a.H = g->H;
+    a.dl = g->cfg.pixel_size_x;
+    a.dm = g->cfg.pixel_size_y;
+    a.lShift = g->cfg.l_shift;
+    a.mShift = g->cfg.m_shift;
+    a.map = layerMap(g, kind);
+    a.passStart = passLayerStart(g, pass);
+    a.nLayersInPass = passLayerStart(g, pass + 1) - a.passStart;
+    a.nLayersPass = g->maxLayersPerPass;
+    a.fresh = fresh ? 1 : 0;
+    a.rowLive = kp.rowLive;
+    a.rowLiveWords = (pl.nbins / size_t(pl.tilesX) + 31) / 32;
+    a.tilesY = pl.tilesY;
+    a.tshift = tshift;
+    a.nImg = kp.nImg;
+    for (int i = 0; i < kMaxImages; ++i) a.img[i] = i < kp.nImg ? kp.img[i] : 0;
+    HWS_TRY(launchColsProject(g, a, kind == 1, layers, g->dSum, g->dComp, g->dISum,
+                              g->dIComp));
+  }
+  return HWS_OK;
 }
 
 hws_status finalizeImages(hws_gridder* g, double mult, float* dReal,
@@ -3033,8 +3050,8 @@ weightSumsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, 
         vw = vw != 0.0f ? 1.0f : 0.0f;
       const double cum = double(vw * imgw[s]);
       if (cum != 0.0) {
-        acc[i][0] = __dadd_rn(acc[i][0], cum);
-        acc[i][1] = __dadd_rn(acc[i][1], double(vw));
+        acc[i][0] = acc[i][0] + cum;
+        acc[i][1] = acc[i][1] + double(vw);
         acc[i][2] = fmax(acc[i][2], cum);
         acc[i][3] += 1.0;
       }
@@ -3065,7 +3082,7 @@ __global__ void weightSumsFinalKernel(const double* __restrict__ partials, int n
   double r = 0.0;
   for (int b = 0; b < nblocks; ++b) {
     const double v = partials[(size_t(b) * kMaxImages + i) * 4 + q];
-    r = q == 2 ? fmax(r, v) : __dadd_rn(r, v);
+    r = q == 2 ? fmax(r, v) : r + v;
   }
   sums[i * 4 + q] = r;
 }

@@ -200,6 +200,11 @@ extern "C" hws_status hws_stage_begin(hws_gridder* g, size_t max_rows, size_t nc
   if (max_rows > (size_t(1) << 30) / nchan)
     return sfail(g, HWS_ERR_UNSUPPORTED, "a plan holds at most 2^30 - 1 (row, channel) samples");
   (void)hipSetDevice(g->cfg.device);
+  // a plan of the previously staged rows reads buffers that may move now
+  if (g->plan.valid && g->plan.uvw == g->dStagedUvw) {
+    g->plan.valid = false;
+    ++g->generation->value;
+  }
   HWS_STRY(grow(g, g->dStagedUvw, g->stagedUvwCap, max_rows * 3, "staged uvw"));
   HWS_STRY(grow(g, g->dStagedWl, g->stagedWlCap, nchan, "staged wavelengths"));
   if (with_visibilities)

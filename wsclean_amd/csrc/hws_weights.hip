@@ -89,8 +89,8 @@ weightGridKernel(const double* __restrict__ uvw, const double* __restrict__ wl, 
       for (int p = 0; p < npol; ++p) {
         float w = wp ? wp[p] : 1.0f;
         if (taper && w != 0.0f) w = 1.0f;
-        run = __dadd_rn(run, double(w));
-        total = __dadd_rn(total, double(w));
+        run = run + double(w);
+        total = total + double(w);
       }
     }
     if (cur >= 0) atomicAdd(grid + cur, run);
@@ -104,7 +104,7 @@ __global__ void addPartialsKernel(const double* __restrict__ partials, size_t n,
                                   double* __restrict__ total) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double s = *total;
-  for (size_t i = 0; i < n; ++i) s = __dadd_rn(s, partials[i]);
+  for (size_t i = 0; i < n; ++i) s = s + partials[i];
   *total = s;
 }
 
@@ -112,26 +112,28 @@ __global__ void addPartialsKernel(const double* __restrict__ partials, size_t n,
 __global__ void __launch_bounds__(kWThreads)
 sumSquaresKernel(const double* __restrict__ grid, size_t n, size_t perBlock,
                  double* __restrict__ partials) {
+#pragma clang fp contract(off)
   __shared__ double red[kWThreads];
   const size_t b0 = size_t(blockIdx.x) * perBlock;
   const size_t b1 = b0 + perBlock < n ? b0 + perBlock : n;
   double s = 0.0;
   for (size_t i = b0 + threadIdx.x; i < b1; i += kWThreads)
-    s = __dadd_rn(s, __dmul_rn(grid[i], grid[i]));
+    s = s + grid[i] * grid[i];
   const double b = blockSum(s, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
 __global__ void __launch_bounds__(kWThreads)
 finishKernel(double* __restrict__ grid, size_t n, int mode, double sSq) {
+#pragma clang fp contract(off)
   const size_t i = size_t(blockIdx.x) * kWThreads + threadIdx.x;
   if (i >= n) return;
   const double v = grid[i];
   double r;
   if (mode == HWS_WEIGHT_BRIGGS)
-    r = __ddiv_rn(1.0, __dadd_rn(1.0, __dmul_rn(v, sSq)));
+    r = 1.0 / (1.0 + v * sSq);
   else if (mode == HWS_WEIGHT_UNIFORM)
-    r = v != 0.0 ? __ddiv_rn(1.0, v) : 0.0;
+    r = v != 0.0 ? 1.0 / v : 0.0;
   else
     r = v != 0.0 ? 1.0 : v;
   grid[i] = r;

@@ -286,7 +286,8 @@ class DeviceGridder(_Context):
 
     def capture_invert(self, vis, multiplication_factor, real_out, imag_out=None):
         """A HIP graph of invert() on the planned samples (single pass);
-        replay with .launch(). Valid until the next plan()."""
+        replay with .launch(). Valid until the next plan(), prepare_w_layers()
+        or set_weighting(), or close(): launch() then raises HwsError."""
         self.check_invert_args(vis, real_out, imag_out)
         gr = C.c_void_p()
         self._check(self.L.hws_graph_capture_invert(self.h, _ptr(vis), multiplication_factor,
@@ -323,7 +324,8 @@ class Graph:
         import torch
         st = stream if stream is not None else torch.cuda.current_stream(
             torch.device("cuda", self.owner.device))
-        self.owner._check(self.owner.L.hws_graph_launch(self.h, C.c_void_p(st.cuda_stream)))
+        # graph errors are reported per thread, not on the context
+        check(self.owner.L.hws_graph_launch(self.h, C.c_void_p(st.cuda_stream)), None)
 
     def close(self):
         if self.h:
