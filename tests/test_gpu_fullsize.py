@@ -22,9 +22,12 @@ them (YY = -2 XX, XY = 0.5 YX, visibilities and models): every floating-point
 operation commutes exactly with a power-of-two scale, so in the deterministic
 batched precisions the GPU's YY and XY outputs must equal -2 and 0.5 times its
 XX and YX outputs bit for bit, which checks that each image slot reads and
-writes only its own data. The plain form sums with global atomics in
-run-dependent order, so there every image is compared with the scaled oracle
-instead.
+writes only its own data. One exception: the FP16 matrix cores that grid the
+mixed precision round their sums asymmetrically about zero, so the YY image
+(a negative factor) matches -2 XX to relative L2 1e-6 rather than bit for bit;
+its predicted visibilities and the XY outputs stay bit-exact. The plain form
+sums with global atomics in run-dependent order, so there every image is
+compared with the scaled oracle instead.
 
 Tolerances (relative L2 against the oracle float restatement): mixed 3e-5,
 as for the small cases (tests/test_gpu_parity.py). The plain form computes
@@ -51,6 +54,7 @@ COMPLEX = (1, 2)
 ARCSEC = np.pi / (180.0 * 3600.0)
 SCALE = {3: -2.0, 1: 0.5}      # image -> factor times its source image
 SOURCE = {3: 0, 1: 2}
+NEG_MULTIPLE_TOL = 1e-6
 ORACLE_THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
@@ -204,12 +208,23 @@ def check_against_oracle(oracle, case, precision, re, im, vis_out, label):
         assert ev < TOL[precision], (label, IMAGES[k], ev)
 
 
+def _same_multiple(a, b, f):
+    # negative factors: the FP16 matrix cores of the mixed precision's grid
+    # kernel round their sums asymmetrically about zero (for negated operands
+    # v_mfma_f32_16x16x32_f16 returns the negated sum but for the last bit of
+    # a few outputs), so an image gridded from negated data matches to the
+    # sums' rounding rather than bit for bit
+    if f < 0:
+        return _rel(a, b) < NEG_MULTIPLE_TOL
+    return np.array_equal(a, b)
+
+
 def check_exact_multiples(re, im, vis_out, label):
     for k, f in SCALE.items():
         s = SOURCE[k]
-        assert np.array_equal(re[k], re[s] * np.float32(f)), (label, IMAGES[k], "real")
+        assert _same_multiple(re[k], re[s] * np.float32(f), f), (label, IMAGES[k], "real")
         if k in COMPLEX:
-            assert np.array_equal(im[k], im[s] * np.float32(f)), (label, IMAGES[k], "imag")
+            assert _same_multiple(im[k], im[s] * np.float32(f), f), (label, IMAGES[k], "imag")
         a, b = vis_out[..., k], vis_out[..., s] * np.complex64(f)
         ok = ~np.isnan(b)
         assert np.array_equal(np.isnan(a), ~ok), (label, IMAGES[k])

@@ -375,6 +375,9 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
       for (int x = 0; x < g->K; ++x)
         tab[size_t(g->OS - i - 1) * g->K + x] = float(g->k1d[size_t(x) * g->OS + i]);
   }
+  float tabMax = 0.0f;
+  for (float v : tab) tabMax = std::max(tabMax, std::fabs(v));
+  (void)std::frexp(tabMax, &g->kernelExp);
   for (size_t i = 0; i < tab.size(); ++i) {
     const __half hi = __float2half(tab[i]);
     const __half lo = __float2half(tab[i] - __half2float(hi));
@@ -407,6 +410,8 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
     g->noMfma = e && e[0] == '1';
     const char* f = std::getenv("HWS_NO_FUSED_PROJECT");
     g->noFusedProject = f && f[0] == '1';
+    const char* h = std::getenv("HWS_NO_F16_MFMA");
+    g->noF16Mfma = h && h[0] == '1';
   }
   // uv tile edge of the binned forms: divides W and H, a multiple of 4, at
   // least K - 1 (a footprint then spans at most two tiles per axis) and with
@@ -469,6 +474,7 @@ void hws_destroy(hws_gridder* g) {
   freeDev(pl.queue);
   freeDev(pl.rot);
   freeDev(pl.imgw);
+  freeDev(pl.cw);
   freeDev(g->dWeightSums);
   freeDev(g->dWeightPartials);
   freeDev(pl.rowKeys);

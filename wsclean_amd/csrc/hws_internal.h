@@ -324,6 +324,10 @@ struct Plan {
   // MSGridderBase keeps it, gridding/msgridderbase.h:502)
   float* imgw = nullptr;
   size_t imgwCapacity = 0;
+  // the weight of every visibility, [sample][image]: visibility weight (after
+  // the weighting mode) times imaging weight, as the grid kernels apply it
+  float* cw = nullptr;
+  size_t cwCapacity = 0;
   bool weighted = false;  // imgw holds this plan's samples
 };
 
@@ -372,6 +376,7 @@ struct hws_gridder {
   std::vector<double> corrX, corrY;  // D[k], k = 0..W/2 (H/2)
   float* dKernel = nullptr;          // [OS][K], tables[p][i]
   __half2* dKernelPair = nullptr;  // [OS][K] (hi, lo) FP16 pairs
+  int kernelExp = 0;               // 2^kernelExp > the table's largest |value|
   float* dCorrXf = nullptr;  // float-rounded D tables (float reference)
   float* dCorrYf = nullptr;
 
@@ -400,6 +405,7 @@ struct hws_gridder {
   // grid with the VALU cell kernel at T = 16 instead of the matrix-core one
   // (HWS_NO_MFMA=1 in the environment; for comparisons)
   bool noMfma = false;
+  bool noF16Mfma = false;  // the mixed precision grids with the FP32 matrix-core kernel
   // run the backward columns pass and the projection apart, and the forward
   // rows pass four layers per workgroup (the forms the fused kernels replace)
   bool noFusedProject = false;

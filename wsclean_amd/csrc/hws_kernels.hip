@@ -263,17 +263,6 @@ __device__ __forceinline__ bool emission(const GeomArgs& ga, const Placement& p,
   return true;
 }
 
-__device__ __forceinline__ Placement placeSample(const GeomArgs& ga, const LayerMap& lm,
-                                                 const double* __restrict__ uvw,
-                                                 const double* __restrict__ wl, size_t i,
-                                                 int nchan) {
-  const size_t row = i / size_t(nchan);
-  const size_t ch = i - row * size_t(nchan);
-  const double lambda = wl[ch];
-  return place(ga, lm, uvw[row * 3] / lambda, uvw[row * 3 + 1] / lambda,
-               uvw[row * 3 + 2] / lambda);
-}
-
 // Sort key of a row: the bin of the first tile its middle channel's footprint
 // touches (0xFFFFFFFF when that sample is not gridded). Rows with equal keys
 // have nearly equal uvw, so their samples fall in the same bins channel by
@@ -760,12 +749,10 @@ struct GridArgs {
   size_t nchan;
   uint32_t chunk;    // records per work item
   const float2* rot;  // rotating contexts: the gridding phasor of every sample
-  // weighted contexts (hws_set_weighting): the imaging weight of every
-  // sample (null: the visibilities are gridded as given), the visibility
-  // weights [sample][image] (null: 1) and the visibility weighting mode
-  const float* imgw;
-  const float* visw;
-  int visWeightMode;
+  // weighted contexts (hws_set_weighting): the weight every visibility is
+  // gridded with, [sample][image] (visibilityWeight, formed by the plan's
+  // weight-sum pass; null: the visibilities are gridded as given)
+  const float* cw;
 };
 
 // The weight readAndWeightVisibilities multiplies a visibility by
@@ -785,8 +772,8 @@ __device__ __forceinline__ float visibilityWeight(const float* __restrict__ imgw
 }
 
 __device__ __forceinline__ float2 weighted(const GridArgs& a, float2 v, uint32_t idx, int img) {
-  if (!a.imgw) return v;
-  const float w = visibilityWeight(a.imgw, a.visw, a.visWeightMode, idx, a.nImgTotal, img);
+  if (!a.cw) return v;
+  const float w = a.cw[size_t(idx) * a.nImgTotal + img];
   return make_float2(v.x * w, v.y * w);
 }
 
@@ -1478,6 +1465,244 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
         const floatx4 re = acc[2 * gi], im = acc[2 * gi + 1];
         if (a.ga.blocked) {
           // rows y0 .. y0 + 3 of one column are 32 contiguous bytes
+          float4* dst = reinterpret_cast<float4*>(base + cellIndex(x, y0, a.ga.W, 1));
+          float4 lo = make_float4(re[0], im[0], re[1], im[1]);
+          float4 hi = make_float4(re[2], im[2], re[3], im[3]);
+          if (ACCUM) {
+            const float4 o0 = dst[0], o1 = dst[1];
+            lo.x += o0.x; lo.y += o0.y; lo.z += o0.z; lo.w += o0.w;
+            hi.x += o1.x; hi.y += o1.y; hi.z += o1.z; hi.w += o1.w;
+          }
+          dst[0] = lo;
+          dst[1] = hi;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float2* dst = base + cellIndex(x, y0 + r, a.ga.W, 0);
+            float2 val = make_float2(re[r], im[r]);
+            if (ACCUM) {
+              const float2 o = *dst;
+              val.x += o.x;
+              val.y += o.y;
+            }
+            *dst = val;
+          }
+        }
+      }
+    }
+  }
+}
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+// Scale of the low parts of split operands: x = h + l 2^-12, with l the FP16
+// rounding of (x - h) 2^12, which keeps l in FP16's normal range wherever h
+// is (the rounding error of h is up to 2^-11 |x|).
+constexpr float kLoScale = 4096.0f, kLoUnscale = 1.0f / 4096.0f;
+// Ranges of the split operands: |A| < 2^14, |B| < 2^13, so that both parts
+// stay below FP16's maximum (65504) and only operands below 2^-28 of the
+// largest fall into the subnormal range.
+constexpr int kAExp = 14, kBExp = 13;
+
+// The mixed precision's tile product on the FP16 matrix cores: every FP32
+// operand x is split into its FP16 rounding h and the scaled FP16 rounding l
+// of its remainder (x = h + l 2^-12 to about 22 bits), and
+// G = Ah Bh + (Ah Bl + Al Bh) 2^-12 (the dropped Al Bl 2^-24 is below 2^-22
+// of each product) is accumulated in FP32 by v_mfma_f32_16x16x32_f16, 32
+// records per instruction, the high and the cross products in separate sums.
+// B = the xKernel values times 2^(kBExp - eB), with 2^eB above the table's
+// largest value; A = v * yKernel, formed in FP32, scaled and split. Each
+// image of a 64-record batch is scaled by a power of two that brings its
+// largest A below 2^kAExp (power-of-two scalings are exact), and its tile
+// product is added to the FP32 sums times the inverse of both scalings.
+// Operand layout: lane i + 16 g supplies A[i][8 g + t] and B[8 g + t][i] in
+// its eight halves t (the same k order on both sides, so records may take
+// any k); D as for gridMfmaKernel. Staging, work queue and output as there.
+template <int NIMG, bool ACCUM, bool ROT>
+__global__ void __launch_bounds__(kMfmaThreads)
+gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
+                  const uint32_t* __restrict__ extraStart, const uint32_t* __restrict__ itemStart,
+                  const uint32_t* __restrict__ itemBin, const uint2* __restrict__ recs,
+                  const float2* __restrict__ vis, const float* __restrict__ ktab,
+                  float2* __restrict__ layers, float2* __restrict__ partials,
+                  uint32_t* __restrict__ queue, int eB) {
+#pragma clang fp contract(off)
+  constexpr int T = 16, T2 = T * T;
+  extern __shared__ float smem[];
+  const int K = a.ga.K, OS = a.ga.OS;
+  const int rowLen = paddedRow(K, T);
+  const int tableWords = (OS * rowLen + 3) & ~3;
+  const float bScale = ldexpf(1.0f, kBExp - eB);
+  float* kt = smem;                                                   // FP32
+  uint32_t* kp = reinterpret_cast<uint32_t*>(smem + tableWords);      // FP16 (h, l)
+  int* stPos = reinterpret_cast<int*>(kp + tableWords);               // [wave][64]
+  float2* stVis = reinterpret_cast<float2*>(stPos + kMfmaWaves * 64);  // [wave][64][NIMG]
+  for (int i = threadIdx.x; i < OS * rowLen; i += blockDim.x) {
+    const int row = i / rowLen, c = i - row * rowLen - (T - 1);
+    float v = 0.0f;
+    if (c >= 0 && c < K) v = ktab[row * K + c];
+    const float vs = v * bScale;
+    const _Float16 h = (_Float16)vs;
+    const _Float16 l = (_Float16)((vs - (float)h) * kLoScale);
+    kt[i] = v;
+    kp[i] = uint32_t(__builtin_bit_cast(uint16_t, h)) |
+            (uint32_t(__builtin_bit_cast(uint16_t, l)) << 16);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;  // A row / B column, k group
+  int* myPos = stPos + wave * 64;
+  float2* myVis = stVis + wave * 64 * NIMG;
+  const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
+  const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
+  const uint32_t nItems = passItems(binLo, binHi, itemStart);
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  auto take = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(queue, 1u);
+    return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+  };
+  uint32_t next = take();
+  while (next < nItems) {
+    const uint32_t item = next;
+    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
+    next = take();
+    const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
+    if (ACCUM && count == 0) continue;
+    floatx4 acc[2 * NIMG];
+#pragma unroll
+    for (int e = 0; e < 2 * NIMG; ++e) acc[e] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t b0 = 0; b0 < count; b0 += 64u) {
+      // stage 64 records (zero samples past the end: their A rows vanish)
+      int pos = 0;
+      float2 v[NIMG];
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) v[gi] = make_float2(0.0f, 0.0f);
+      const uint32_t it = b0 + uint32_t(lane);
+      if (it < count) {
+        const uint2 rec = recs[start + it];
+        const uint32_t idx = rec.x & 0x3FFFFFFFu;
+        const bool conj = (rec.x >> 31) != 0;
+        const int kk = int(rec.y >> 12);
+        const int ky = kk / OS, kx = kk - ky * OS;
+        const int xs = int(rec.y & 63u), ys = int((rec.y >> 6) & 63u);
+        pos = (kx * rowLen + (T - 1) + (K - 1) - xs) |
+              ((ky * rowLen + (T - 1) + (K - 1) - ys) << 16);
+        float2 ph = make_float2(1.0f, 0.0f);
+        if (ROT) ph = a.rot[idx];
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          float2 x = weighted(a, vis[size_t(idx) * a.nImgTotal + a.imgs.idx[gi]], idx,
+                              a.imgs.idx[gi]);
+          if (ROT) x = rotateBy(x, ph);
+          if (conj) x.y = -x.y;
+          v[gi] = x;
+        }
+      }
+      // each image's scale: 2^e above its largest visibility component in
+      // the batch, so that |v yKernel| 2^(kAExp - e - eB) < 2^kAExp (a scale
+      // per image keeps images that are power-of-two multiples of each
+      // other exact multiples, FP16 subnormals included)
+      float up[NIMG];
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        float m = fmaxf(fabsf(v[gi].x), fabsf(v[gi].y));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        int e = 0;
+        (void)frexpf(m, &e);  // m < 2^e (0 for m = 0)
+        const float down = ldexpf(1.0f, kAExp - e - eB);
+        up[gi] = ldexpf(1.0f, e + 2 * eB - kAExp - kBExp);
+        v[gi].x = v[gi].x * down;
+        v[gi].y = v[gi].y * down;
+      }
+      myPos[lane] = pos;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) myVis[lane * NIMG + gi] = v[gi];
+      waveSync();
+      floatx4 d[2 * NIMG], dx[2 * NIMG];  // high products, cross products
+#pragma unroll
+      for (int c = 0; c < 2 * NIMG; ++c) {
+        d[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        dx[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+#pragma unroll
+      for (int G = 0; G < 2; ++G) {
+        int ps[8];
+        float kyv[8];
+        uint32_t pr[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) ps[t] = myPos[32 * G + 8 * lg + t];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          // B[k][x = li] = xKernel, A[y = li][k] = v * yKernel
+          // (yKernelValue * xKernel[i], gridding/wstackinggridder.cpp:574)
+          pr[t] = kp[(ps[t] & 0xFFFF) + li];
+          kyv[t] = kt[(ps[t] >> 16) + li];
+        }
+        half8 bh, bl;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          bh[t] = __builtin_bit_cast(_Float16, uint16_t(pr[t] & 0xFFFFu));
+          bl[t] = __builtin_bit_cast(_Float16, uint16_t(pr[t] >> 16));
+        }
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi) {
+          half8 ah[2], al[2];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const float2 vv = myVis[(32 * G + 8 * lg + t) * NIMG + gi];
+            float xr = vv.x * kyv[t], xi = vv.y * kyv[t];
+            // the FP32 products as values: the compiler otherwise rounds the
+            // exact product to FP16 (v_fma_mixlo_f16) for one use of h and
+            // the FP32 product for the other, and where those differ (near
+            // FP16 ties) the split loses an FP16 unit
+            __asm__("" : "+v"(xr), "+v"(xi));
+            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+            ah[0][t] = hr;
+            ah[1][t] = hi;
+            // (x - h) is exact in FP32, and so is its scaling
+            al[0][t] = (_Float16)((xr - (float)hr) * kLoScale);
+            al[1][t] = (_Float16)((xi - (float)hi) * kLoScale);
+          }
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            d[2 * gi + c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bh, d[2 * gi + c],
+                                                                   0, 0, 0);
+            dx[2 * gi + c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bl, dx[2 * gi + c],
+                                                                    0, 0, 0);
+            dx[2 * gi + c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[c], bh, dx[2 * gi + c],
+                                                                    0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 2 * NIMG; ++c) {
+        const float u = up[c / 2], ux = u * kLoUnscale;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[c][r] = acc[c][r] + (d[c][r] * u + dx[c][r] * ux);
+      }
+      waveSync();  // the staging is rewritten next batch
+    }
+    // lane owns column li, rows 4 lg .. 4 lg + 3
+    if (wi.j > 0) {
+      float2* dst = partials + size_t(extraStart[bin] + wi.j - 1u) * size_t(NIMG * T2);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int gi = 0; gi < NIMG; ++gi)
+          dst[gi * T2 + (4 * lg + r) * T + li] = make_float2(acc[2 * gi][r], acc[2 * gi + 1][r]);
+    } else {
+      const int layer = int(bin / uint32_t(a.tilesPerLayer));
+      const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
+      const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+      const int x = tx * T + li, y0 = ty * T + 4 * lg;
+#pragma unroll
+      for (int gi = 0; gi < NIMG; ++gi) {
+        float2* base = layers + (size_t(a.imgs.idx[gi]) * a.nLayersPass +
+                                 size_t(layer - a.passStart)) * HW;
+        const floatx4 re = acc[2 * gi], im = acc[2 * gi + 1];
+        if (a.ga.blocked) {
           float4* dst = reinterpret_cast<float4*>(base + cellIndex(x, y0, a.ga.W, 1));
           float4 lo = make_float4(re[0], im[0], re[1], im[1]);
           float4 hi = make_float4(re[2], im[2], re[3], im[3]);
@@ -2220,6 +2445,11 @@ bool mfmaKernelUsable(const hws_gridder* g) {
   return g->plan.tileSize == 16 && cellTablePadded(g->K, g->OS, 16) && !g->noMfma;
 }
 
+// The mixed precision grids on the FP16 matrix cores (split operands).
+bool mfmaF16(const hws_gridder* g) {
+  return mfmaKernelUsable(g) && g->cfg.precision == HWS_PRECISION_MIXED && !g->noF16Mfma;
+}
+
 size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
   const int T = g->tileSize;
   const size_t kt = size_t((g->OS * g->K + 3) & ~3) * sizeof(float);
@@ -2229,7 +2459,9 @@ size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
            size_t(kThreads / 64) * kDegridQueue * sizeof(uint2);
   }
   if (mfmaKernelUsable(g)) {
-    const size_t table = size_t((g->OS * paddedRow(g->K, T) + 3) & ~3) * sizeof(float);
+    // the FP32 table, and for the FP16 kernel the (hi, lo) pairs beside it
+    const size_t table = size_t((g->OS * paddedRow(g->K, T) + 3) & ~3) * sizeof(float) *
+                         (mfmaF16(g) ? 2 : 1);
     return table + kMfmaWaves * 64 * (sizeof(int) + size_t(nimg) * sizeof(float2));
   }
   if (cellKernelTile(T)) {
@@ -2535,9 +2767,7 @@ GridArgs gridArgs(const hws_gridder* g, int pass) {
   a.nchan = pl.nchan;
   a.chunk = pl.chunk;
   a.rot = pl.rot;
-  a.imgw = (g->weights && pl.weighted) ? pl.imgw : nullptr;
-  a.visw = g->dVisWeights;
-  a.visWeightMode = g->visWeightMode;
+  a.cw = (g->weights && pl.weighted) ? pl.cw : nullptr;
   return a;
 }
 }  // namespace
@@ -2588,17 +2818,32 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
             1, std::min<uint64_t>((kp.nWorkItems + kMfmaWaves - 1) / kMfmaWaves,
                                   uint64_t(persistentBlocks(g, 8)))));
         hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, g->stream, pl.queue, 0u, size_t(1));
+        const bool f16 = mfmaF16(g);
 #define HWS_MFMA_GO(N, ACC, ROT)                                                               \
-  hipLaunchKernelGGL((gridMfmaKernel<N, ACC, ROT>), dim3(mblocks), dim3(kMfmaThreads), lds,     \
-                     g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.itemBin,        \
-                     kp.recSorted, vis, g->dKernel, khalf, pl.uvw, pl.wavelengths, layers,      \
-                     kp.partials, pl.queue)
+  if (f16)                                                                                     \
+    hipLaunchKernelGGL((gridMfmaF16Kernel<N, ACC, ROT>), dim3(mblocks), dim3(kMfmaThreads),     \
+                       lds, g->stream, a, kp.binStart, kp.extraStart, kp.itemStart,             \
+                       kp.itemBin, kp.recSorted, vis, g->dKernel, layers, kp.partials,          \
+                       pl.queue, g->kernelExp);                                                 \
+  else                                                                                         \
+    hipLaunchKernelGGL((gridMfmaKernel<N, ACC, ROT>), dim3(mblocks), dim3(kMfmaThreads), lds,   \
+                       g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.itemBin,      \
+                       kp.recSorted, vis, g->dKernel, khalf, pl.uvw, pl.wavelengths, layers,    \
+                       kp.partials, pl.queue)
 #define HWS_MFMA_CASE(N)                                                                       \
   case N:                                                                                      \
     if (accumulate) {                                                                          \
-      if (a.ga.rotate) HWS_MFMA_GO(N, true, true); else HWS_MFMA_GO(N, true, false);          \
+      if (a.ga.rotate) {                                                                       \
+        HWS_MFMA_GO(N, true, true);                                                            \
+      } else {                                                                                 \
+        HWS_MFMA_GO(N, true, false);                                                           \
+      }                                                                                        \
     } else {                                                                                   \
-      if (a.ga.rotate) HWS_MFMA_GO(N, false, true); else HWS_MFMA_GO(N, false, false);        \
+      if (a.ga.rotate) {                                                                       \
+        HWS_MFMA_GO(N, false, true);                                                           \
+      } else {                                                                                 \
+        HWS_MFMA_GO(N, false, false);                                                          \
+      }                                                                                        \
     }                                                                                          \
     break;
         switch (n) {
@@ -3029,7 +3274,7 @@ struct WeightSumArgs {
 __global__ void __launch_bounds__(kWeightThreads)
 weightSumsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
                  size_t nchan, const float* __restrict__ imgw, const float* __restrict__ visw,
-                 WeightSumArgs a, double* __restrict__ partials) {
+                 WeightSumArgs a, double* __restrict__ partials, float* __restrict__ cw) {
   __shared__ double red[kWeightThreads];
   double acc[kMaxImages][4];
 #pragma unroll
@@ -3042,13 +3287,15 @@ weightSumsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, 
     const double w = uvw[row * 3 + 2] / wl[ch];
     const bool sel[2] = {layerOf(a.maps[0], w) >= 0, layerOf(a.maps[1], w) >= 0};
     for (int i = 0; i < a.nImg; ++i) {
-      if (!sel[a.kindOf[i]]) continue;
       float vw = visw ? visw[s * size_t(a.nImg) + size_t(i)] : 1.0f;
       if (a.mode == 1)
         vw = vw * vw;
       else if (a.mode == 2)
         vw = vw != 0.0f ? 1.0f : 0.0f;
-      const double cum = double(vw * imgw[s]);
+      const float cwf = vw * imgw[s];
+      cw[s * size_t(a.nImg) + size_t(i)] = cwf;  // what the grid kernels multiply by
+      if (!sel[a.kindOf[i]]) continue;
+      const double cum = double(cwf);
       if (cum != 0.0) {
         acc[i][0] = acc[i][0] + cum;
         acc[i][1] = acc[i][1] + double(vw);
@@ -3110,6 +3357,7 @@ hws_status planWeights(hws_gridder* g) {
   }
   if (n > 0) {
     HWS_TRY(ensureAlloc(g, pl.imgw, pl.imgwCapacity, n, "plan imaging weights"));
+    HWS_TRY(ensureAlloc(g, pl.cw, pl.cwCapacity, n * size_t(g->nImg), "plan gridding weights"));
     hipLaunchKernelGGL(imagingWeightsKernel, dim3(unsigned(std::min<size_t>(
                                                  (n + kWeightThreads - 1) / kWeightThreads,
                                                  size_t(8192)))),
@@ -3125,7 +3373,7 @@ hws_status planWeights(hws_gridder* g) {
   a.mode = g->visWeightMode;
   hipLaunchKernelGGL(weightSumsKernel, dim3(nblocks), dim3(kWeightThreads), 0, g->stream,
                      pl.uvw, pl.wavelengths, pl.nrow, pl.nchan, pl.imgw, g->dVisWeights, a,
-                     g->dWeightPartials);
+                     g->dWeightPartials, pl.cw);
   HWS_TRY(checkHip(g, hipGetLastError(), "weight sums launch"));
   hipLaunchKernelGGL(weightSumsFinalKernel, dim3(1), dim3(64), 0, g->stream,
                      g->dWeightPartials, nblocks, g->nImg, g->dWeightSums);

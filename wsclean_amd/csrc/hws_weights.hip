@@ -99,13 +99,19 @@ weightGridKernel(const double* __restrict__ uvw, const double* __restrict__ wl, 
   if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
-// Adds n partial sums to *total in index order (one thread).
-__global__ void addPartialsKernel(const double* __restrict__ partials, size_t n,
-                                  double* __restrict__ total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s = *total;
-  for (size_t i = 0; i < n; ++i) s = s + partials[i];
-  *total = s;
+// Adds n partial sums to *total in a fixed order (one workgroup): thread t
+// sums its contiguous share in index order, the shares are added by the
+// fixed tree of blockSum, then the result to *total.
+__global__ void __launch_bounds__(kWThreads)
+addPartialsKernel(const double* __restrict__ partials, size_t n, double* __restrict__ total) {
+#pragma clang fp contract(off)
+  __shared__ double red[kWThreads];
+  const size_t per = (n + kWThreads - 1) / kWThreads;
+  const size_t i0 = size_t(threadIdx.x) * per, i1 = i0 + per < n ? i0 + per : n;
+  double s = 0.0;
+  for (size_t i = i0; i < i1; ++i) s = s + partials[i];
+  const double b = blockSum(s, red);
+  if (threadIdx.x == 0) *total = *total + b;
 }
 
 // Per-block partial sums of grid[i]^2 (FinishGridding's avgW numerator).
@@ -235,7 +241,7 @@ extern "C" hws_status hws_weights_grid_device(hws_weights* w, void* hip_stream, 
   hipLaunchKernelGGL(weightGridKernel, dim3(unsigned(blocks)), dim3(kWThreads), 0, stream,
                      d_uvw_m, d_wavelengths, nrow, int(nchan), int(npol), d_weights,
                      w->weightsAsTaper, weightGridArgs(w), w->dGrid, w->dPartials);
-  hipLaunchKernelGGL(addPartialsKernel, dim3(1), dim3(64), 0, stream, w->dPartials, blocks,
+  hipLaunchKernelGGL(addPartialsKernel, dim3(1), dim3(kWThreads), 0, stream, w->dPartials, blocks,
                      w->dTotal);
   return checkHip(nullptr, hipGetLastError(), "weight gridding launch");
 }
