@@ -477,8 +477,13 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass);
 // NaN for every sample of a kind that no record grids (with degridBinned
 // over all passes, every value of the predicted visibilities is written).
 hws_status markSkipped(hws_gridder* g, float* dVisOut);
+// Exclusive scan of n 32-bit values into out[0..n] (out[n] = the total) on
+// the context's stream; scratch holds scanScratchSize(n) values.
+size_t scanScratchSize(size_t n);
+hws_status exclusiveScan(hws_gridder* g, const uint32_t* in, uint32_t* out, size_t n,
+                         uint32_t* scratch);
 // Sorts n records (and their 32-bit keys) by key bits [0, endBit) on the
-// context's stream; on return `recs` holds the sorted records (the two record
+// context's stream (stable); on return `recs` holds the sorted records (the two record
 // buffers may have been swapped).
 hws_status sortRecords(hws_gridder* g, uint32_t*& keys, uint32_t*& keysAlt, uint2*& recs,
                        uint2*& recsAlt, size_t n, int endBit);

@@ -2364,6 +2364,8 @@ unsigned blocksFor(size_t n, int threads = kThreads) {
   return unsigned((n + threads - 1) / threads);
 }
 
+}  // namespace
+
 // Exclusive scan of n values into out[0..n] (out[n] = total); scratch holds
 // the block sums of every level (scanScratchSize(n) values).
 size_t scanScratchSize(size_t n) {
@@ -2395,6 +2397,8 @@ hws_status exclusiveScan(hws_gridder* g, const uint32_t* in, uint32_t* out,
   hipLaunchKernelGGL(scanTotalKernel, dim3(1), dim3(64), 0, g->stream, in, out, n);
   return checkHip(g, hipGetLastError(), "scan total");
 }
+
+namespace {
 
 GeomArgs geomArgs(const hws_gridder* g) {
   GeomArgs ga;
