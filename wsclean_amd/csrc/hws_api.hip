@@ -473,7 +473,6 @@ void hws_destroy(hws_gridder* g) {
   freeDev(pl.counters);
   freeDev(pl.queue);
   freeDev(pl.rot);
-  freeDev(pl.imgw);
   freeDev(pl.cw);
   freeDev(g->dWeightSums);
   freeDev(g->dWeightPartials);

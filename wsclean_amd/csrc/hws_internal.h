@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -320,15 +321,13 @@ struct Plan {
          rowOrderAltCapacity = 0;
   void* sortTemp = nullptr;  // radix sort scratch
   size_t sortTempCapacity = 0;
-  // weighted contexts: the imaging weight of every sample (float, as
-  // MSGridderBase keeps it, gridding/msgridderbase.h:502)
-  float* imgw = nullptr;
-  size_t imgwCapacity = 0;
-  // the weight of every visibility, [sample][image]: visibility weight (after
-  // the weighting mode) times imaging weight, as the grid kernels apply it
+  // weighted contexts: the weight of every visibility, [sample][image]:
+  // visibility weight (after the weighting mode) times the sample's imaging
+  // weight (the float MSGridderBase keeps, gridding/msgridderbase.h:502), as
+  // the grid kernels apply it
   float* cw = nullptr;
   size_t cwCapacity = 0;
-  bool weighted = false;  // imgw holds this plan's samples
+  bool weighted = false;  // cw holds this plan's samples
 };
 
 // Version of what a captured graph of a context bakes in: the plan and its

@@ -756,21 +756,8 @@ struct GridArgs {
 };
 
 // The weight readAndWeightVisibilities multiplies a visibility by
-// (gridding/msgridderbase.cpp:567-589, 640-659): the visibility weight after
-// the visibility weighting mode (as given, squared, or 1 where non-zero)
-// times the sample's imaging weight, a float product as the reference forms
-// it.
-__device__ __forceinline__ float visibilityWeight(const float* __restrict__ imgw,
-                                                  const float* __restrict__ visw, int mode,
-                                                  size_t sample, int nImgTotal, int img) {
-  float w = visw ? visw[sample * size_t(nImgTotal) + size_t(img)] : 1.0f;
-  if (mode == 1)
-    w = w * w;
-  else if (mode == 2)
-    w = w != 0.0f ? 1.0f : 0.0f;
-  return w * imgw[sample];
-}
-
+// (gridding/msgridderbase.cpp:567-589, 640-659), formed once per plan by
+// weightSumsKernel.
 __device__ __forceinline__ float2 weighted(const GridArgs& a, float2 v, uint32_t idx, int img) {
   if (!a.cw) return v;
   const float w = a.cw[size_t(idx) * a.nImgTotal + img];
@@ -1992,9 +1979,7 @@ struct PlainArgs {
   int ch;
   size_t nrow, nchan;
   int blocked;
-  const float* imgw;  // as GridArgs
-  const float* visw;
-  int visWeightMode;
+  const float* cw;  // as GridArgs
 };
 
 struct PlainPlacement {
@@ -2062,9 +2047,8 @@ gridPlainKernel(PlainArgs a, const double* __restrict__ uvw,
   const PlainPlacement p = placePlain(a, uvw, wl, r, -1.0f);
   if (!p.ok) return;
   float2 v = vis[(r * a.nchan + a.ch) * a.nImgTotal + a.img];
-  if (a.imgw) {
-    const float w = visibilityWeight(a.imgw, a.visw, a.visWeightMode, r * a.nchan + a.ch,
-                                     a.nImgTotal, a.img);
+  if (a.cw) {
+    const float w = a.cw[(r * a.nchan + a.ch) * a.nImgTotal + a.img];
     v = make_float2(v.x * w, v.y * w);
   }
   if (a.rotate) v = make_float2(v.x * p.rotRe - v.y * p.rotIm, v.x * p.rotIm + v.y * p.rotRe);
@@ -3061,9 +3045,7 @@ PlainArgs plainArgs(const hws_gridder* g, int img, int pass, size_t nrow,
   a.nrow = nrow;
   a.nchan = nchan;
   a.blocked = g->blocked;
-  a.imgw = (g->weights && g->plan.weighted) ? g->plan.imgw : nullptr;
-  a.visw = g->dVisWeights;
-  a.visWeightMode = g->visWeightMode;
+  a.cw = (g->weights && g->plan.weighted) ? g->plan.cw : nullptr;
   return a;
 }
 }  // namespace
@@ -3248,67 +3230,95 @@ namespace {
 
 constexpr int kWeightThreads = 256;
 
-// The imaging weight of every (row, channel) sample: GetWeight at
-// (u, v) = uvw / lambda (gridding/msgridderbase.cpp:584-589), stored as the
-// float the reference keeps.
-__global__ void __launch_bounds__(kWeightThreads)
-imagingWeightsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
-                     size_t nchan, WeightGridArgs wg, float* __restrict__ out) {
-  const size_t n = nrow * nchan;
-  for (size_t s = size_t(blockIdx.x) * kWeightThreads + threadIdx.x; s < n;
-       s += size_t(gridDim.x) * kWeightThreads) {
-    const size_t row = s / nchan, ch = s - row * nchan;
-    const double lambda = wl[ch];
-    out[s] = float(imagingWeightAt(wg, uvw[row * 3] / lambda, uvw[row * 3 + 1] / lambda));
-  }
-}
-
 struct WeightSumArgs {
   LayerMap maps[2];
-  int nImg;
   int kindOf[kMaxImages];
   int mode;
+  int viswVector;  // the visibility weights are aligned for vector loads
+  WeightGridArgs wg;
 };
 
-// Per-block partial sums per image of the counters readAndWeightVisibilities
-// keeps (gridding/msgridderbase.cpp:640-659), over the samples whose w-layer
-// the image kind grids (the others are deselected, :563-565, and add
-// nothing): [block][image][0 total weight, 1 visibility weight sum, 2 max
-// gridded weight, 3 gridded count].
+// Per (row, channel) sample: the imaging weight GetWeight at (u, v) =
+// uvw / lambda (gridding/msgridderbase.cpp:584-589, the float the reference
+// keeps), and per image the weight the grid kernels multiply the visibility
+// by, cw = visibilityWeight * imagingWeight (:567-589, the visibility weight
+// after the weighting mode: as given, squared, or 1 where non-zero; a float
+// product as the reference forms it). Per-block partial sums per image of
+// the counters readAndWeightVisibilities keeps (:640-659), over the samples
+// whose w-layer the image kind grids (the others are deselected, :563-565,
+// and add nothing): [block][image][0 total weight, 1 visibility weight sum,
+// 2 max gridded weight, 3 gridded count]. NIMG images per sample, read and
+// written as one vector where NIMG is 2 or 4.
+template <int NIMG>
 __global__ void __launch_bounds__(kWeightThreads)
-weightSumsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, size_t nrow,
-                 size_t nchan, const float* __restrict__ imgw, const float* __restrict__ visw,
-                 WeightSumArgs a, double* __restrict__ partials, float* __restrict__ cw) {
+weightSumsKernel(const double* __restrict__ uvw, const double* __restrict__ wl, uint32_t nrow,
+                 uint32_t nchan, const float* __restrict__ visw, WeightSumArgs a,
+                 double* __restrict__ partials, float* __restrict__ cw) {
+#pragma clang fp contract(off)
   __shared__ double red[kWeightThreads];
-  double acc[kMaxImages][4];
+  double acc[NIMG][4];
 #pragma unroll
-  for (int i = 0; i < kMaxImages; ++i) acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0.0;
-  const size_t n = nrow * nchan;
-  const size_t per = (n + gridDim.x - 1) / gridDim.x;
-  const size_t s0 = size_t(blockIdx.x) * per, s1 = s0 + per < n ? s0 + per : n;
-  for (size_t s = s0 + threadIdx.x; s < s1; s += kWeightThreads) {
-    const size_t row = s / nchan, ch = s - row * nchan;
-    const double w = uvw[row * 3 + 2] / wl[ch];
+  for (int i = 0; i < NIMG; ++i) acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0.0;
+  const uint32_t n = nrow * nchan;
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t s0 = blockIdx.x * per, s1 = s0 + per < n ? s0 + per : n;
+  using Vec = typename std::conditional<NIMG == 4, float4,
+                                        typename std::conditional<NIMG == 2, float2,
+                                                                  float>::type>::type;
+  constexpr bool kVec = NIMG == 2 || NIMG == 4;
+  for (uint32_t s = s0 + threadIdx.x; s < s1; s += kWeightThreads) {
+    const uint32_t row = s / nchan, ch = s - row * nchan;
+    const double lambda = wl[ch];
+    const double u = uvw[size_t(row) * 3] / lambda, v = uvw[size_t(row) * 3 + 1] / lambda;
+    const double w = uvw[size_t(row) * 3 + 2] / lambda;
+    const float imgw = float(imagingWeightAt(a.wg, u, v));
     const bool sel[2] = {layerOf(a.maps[0], w) >= 0, layerOf(a.maps[1], w) >= 0};
-    for (int i = 0; i < a.nImg; ++i) {
-      float vw = visw ? visw[s * size_t(a.nImg) + size_t(i)] : 1.0f;
+    float vw[NIMG], c[NIMG];
+    if (visw) {
+      if (kVec && a.viswVector) {
+        const Vec x = reinterpret_cast<const Vec*>(visw)[s];
+        const float* xf = reinterpret_cast<const float*>(&x);
+#pragma unroll
+        for (int i = 0; i < NIMG; ++i) vw[i] = xf[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < NIMG; ++i) vw[i] = visw[size_t(s) * NIMG + i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NIMG; ++i) vw[i] = 1.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < NIMG; ++i) {
       if (a.mode == 1)
-        vw = vw * vw;
+        vw[i] = vw[i] * vw[i];
       else if (a.mode == 2)
-        vw = vw != 0.0f ? 1.0f : 0.0f;
-      const float cwf = vw * imgw[s];
-      cw[s * size_t(a.nImg) + size_t(i)] = cwf;  // what the grid kernels multiply by
+        vw[i] = vw[i] != 0.0f ? 1.0f : 0.0f;
+      c[i] = vw[i] * imgw;
+    }
+    if constexpr (kVec) {
+      Vec y;
+      float* yf = reinterpret_cast<float*>(&y);
+#pragma unroll
+      for (int i = 0; i < NIMG; ++i) yf[i] = c[i];
+      reinterpret_cast<Vec*>(cw)[s] = y;
+    } else {
+#pragma unroll
+      for (int i = 0; i < NIMG; ++i) cw[size_t(s) * NIMG + i] = c[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NIMG; ++i) {
       if (!sel[a.kindOf[i]]) continue;
-      const double cum = double(cwf);
+      const double cum = double(c[i]);
       if (cum != 0.0) {
         acc[i][0] = acc[i][0] + cum;
-        acc[i][1] = acc[i][1] + double(vw);
+        acc[i][1] = acc[i][1] + double(vw[i]);
         acc[i][2] = fmax(acc[i][2], cum);
         acc[i][3] += 1.0;
       }
     }
   }
-  for (int i = 0; i < a.nImg; ++i)
+  for (int i = 0; i < NIMG; ++i)
     for (int q = 0; q < 4; ++q) {
       red[threadIdx.x] = acc[i][q];
       __syncthreads();
@@ -3359,25 +3369,32 @@ hws_status planWeights(hws_gridder* g) {
                      "weight partial sums"));
     g->weightPartialCapacity = size_t(nblocks) * kMaxImages * 4;
   }
-  if (n > 0) {
-    HWS_TRY(ensureAlloc(g, pl.imgw, pl.imgwCapacity, n, "plan imaging weights"));
+  if (n > 0)
     HWS_TRY(ensureAlloc(g, pl.cw, pl.cwCapacity, n * size_t(g->nImg), "plan gridding weights"));
-    hipLaunchKernelGGL(imagingWeightsKernel, dim3(unsigned(std::min<size_t>(
-                                                 (n + kWeightThreads - 1) / kWeightThreads,
-                                                 size_t(8192)))),
-                       dim3(kWeightThreads), 0, g->stream, pl.uvw, pl.wavelengths, pl.nrow,
-                       pl.nchan, weightGridArgs(g->weights), pl.imgw);
-    HWS_TRY(checkHip(g, hipGetLastError(), "imaging weights launch"));
-  }
   WeightSumArgs a{};
   a.maps[0] = layerMap(g, 0);
   a.maps[1] = layerMap(g, 1);
-  a.nImg = g->nImg;
   for (int i = 0; i < g->nImg; ++i) a.kindOf[i] = g->isComplex[i] ? 1 : 0;
   a.mode = g->visWeightMode;
-  hipLaunchKernelGGL(weightSumsKernel, dim3(nblocks), dim3(kWeightThreads), 0, g->stream,
-                     pl.uvw, pl.wavelengths, pl.nrow, pl.nchan, pl.imgw, g->dVisWeights, a,
-                     g->dWeightPartials, pl.cw);
+  a.wg = weightGridArgs(g->weights);
+  a.viswVector = (reinterpret_cast<uintptr_t>(g->dVisWeights) %
+                  (sizeof(float) * size_t(g->nImg == 3 ? 1 : g->nImg))) == 0;
+  const uint32_t nr = uint32_t(pl.nrow), nc = uint32_t(pl.nchan);
+  switch (g->nImg) {
+#define HWS_WSUMS(N)                                                                         \
+  case N:                                                                                    \
+    hipLaunchKernelGGL((weightSumsKernel<N>), dim3(nblocks), dim3(kWeightThreads), 0,         \
+                       g->stream, pl.uvw, pl.wavelengths, nr, nc, g->dVisWeights, a,          \
+                       g->dWeightPartials, pl.cw);                                           \
+    break;
+    HWS_WSUMS(1)
+    HWS_WSUMS(2)
+    HWS_WSUMS(3)
+    HWS_WSUMS(4)
+#undef HWS_WSUMS
+    default:
+      return setLastError(g, HWS_ERR_UNSUPPORTED, "images per context");
+  }
   HWS_TRY(checkHip(g, hipGetLastError(), "weight sums launch"));
   hipLaunchKernelGGL(weightSumsFinalKernel, dim3(1), dim3(64), 0, g->stream,
                      g->dWeightPartials, nblocks, g->nImg, g->dWeightSums);
