@@ -51,6 +51,8 @@ def parse():
                    help="first copy BYTES device-to-device (the known-size copy against which "
                         "hardware byte counters of a profile are scaled)")
     p.add_argument("--no-facets", action="store_true", help="skip the facet-batch leg")
+    p.add_argument("--no-plugin", action="store_true",
+                   help="skip the plugin leg (host rows staged over the bus, host outputs)")
     p.add_argument("--facets-only", action="store_true",
                    help="run only the facet-batch leg and print its JSON object")
     p.add_argument("--facet-scale", type=float, default=0.25,
@@ -386,6 +388,77 @@ def facet_leg(args, dev):
     }
 
 
+def plugin_leg(w, steps=1, batch_rows=65536):
+    """The drop-in plugin's path at this workload, host memory at both ends:
+    what HipWSMSGridder does per WSClean Invert and Predict call once its
+    reader has produced the (pre-weighted) rows. Through the staged C ABI the
+    rows go batch by batch into two pinned host buffers and cross the bus
+    asynchronously while the next batch is copied (hws_stage_rows), are
+    planned once per call (hws_stage_end), inverted or predicted on the
+    device, and the four images or the predicted visibilities come back to
+    host memory. One step = invert (stage uvw + visibilities, plan, invert,
+    images to host) + predict (stage uvw, plan, models from host, predict,
+    visibilities to host). The visibilities are the workload's, unweighted
+    (the plugin weights its rows on the host before staging them)."""
+    import ctypes as C
+    import wsclean_amd
+    uvw = np.ascontiguousarray(w.uvw.cpu().numpy())
+    wl = np.ascontiguousarray(w.wl.cpu().numpy())
+    vis = np.ascontiguousarray(w.vis.cpu().numpy())
+    out = np.empty_like(vis)
+    real = np.zeros((4, w.H, w.W), np.float32)
+    imag = np.zeros_like(real)
+    g = wsclean_amd.WStackingGridder(w.W, w.H, w.PIXEL, w.PIXEL, 7, 63, precision="mixed",
+                                     n_images=4, complex_images=w.COMPLEX)
+    g.prepare_w_layers(Workload.NW, 0.0, w.min_w, w.max_w)
+    L, h = g.L, g.h
+    pd, pf = C.POINTER(C.c_double), C.POINTER(C.c_float)
+    nrow, nchan = w.nrow, w.nchan
+    phases = {"invert_stage_s": 0.0, "invert_s": 0.0, "predict_stage_s": 0.0, "predict_s": 0.0}
+
+    def stage(with_vis):
+        g._check(L.hws_stage_begin(h, nrow, nchan, wl.ctypes.data_as(pd), int(with_vis)))
+        for r0 in range(0, nrow, batch_rows):
+            r1 = min(nrow, r0 + batch_rows)
+            g._check(L.hws_stage_rows(h, r1 - r0, uvw[r0:r1].ctypes.data_as(pd),
+                                      vis[r0:r1].ctypes.data_as(pf) if with_vis else None))
+        g._check(L.hws_stage_end(h))
+
+    def step(record):
+        t = time.perf_counter()
+        stage(True)
+        t1 = time.perf_counter()
+        g._check(L.hws_invert_staged(h, 1.0, w.W, w.H, w.W, w.H, real.ctypes.data_as(pf),
+                                     imag.ctypes.data_as(pf)))
+        t2 = time.perf_counter()
+        stage(False)
+        t3 = time.perf_counter()
+        g._check(L.hws_predict_staged(h, real.ctypes.data_as(pf), imag.ctypes.data_as(pf),
+                                      w.W, w.H, w.W, w.H, out.ctypes.data_as(pf)))
+        t4 = time.perf_counter()
+        if record:
+            phases["invert_stage_s"] += t1 - t
+            phases["invert_s"] += t2 - t1
+            phases["predict_stage_s"] += t3 - t2
+            phases["predict_s"] += t4 - t3
+
+    step(False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    elapsed = (time.perf_counter() - t0) / steps
+    g.close()
+    finite = bool(np.isfinite(real).all() and np.isfinite(imag[list(w.COMPLEX)]).all())
+    return {"value": w.nvis / elapsed, "unit": "visibilities/s", "s_per_step": elapsed,
+            "steps": steps, "phases_s": {k: round(v / steps, 3) for k, v in phases.items()},
+            "bytes_to_device_per_step": int(2 * uvw.nbytes + vis.nbytes + real.nbytes * 2),
+            "bytes_to_host_per_step": int(out.nbytes + real.nbytes * 2),
+            "images_finite": finite,
+            "what": "staged host rows -> plan -> invert -> host images; staged uvw -> plan -> "
+                    "host models -> predict -> host visibilities (mixed precision, unweighted, "
+                    "batches of %d rows)" % batch_rows}
+
+
 def traffic_from_profiles():
     """HBM bytes per step of each kernel class, measured with rocprofv3 --pmc
     FETCH_SIZE and WRITE_SIZE over one profiled step and scaled by the same
@@ -456,8 +529,9 @@ def main():
     #   columns + projection (fftColsProjectKernel): 8 B read per cell of a live
     #     tile row; the accumulators read and written once (sum and
     #     compensation, 8 B each way; the imaginary ones for every image);
-    #   grid (gridMfmaKernel): 245 FP32 flops per gridded visibility (49 taps,
-    #     5 flops), against the FP32 matrix peak;
+    #   grid (gridMfmaF16Kernel): 245 FP32 flops per gridded visibility (49
+    #     taps, 5 flops), against the FP32 matrix peak (the kernel forms them as
+    #     three FP16 products of split operands with FP32 sums);
     #   degrid (degridTilesKernel): 245 FP32 flops per predicted visibility,
     #     against the FP32 vector peak;
     #   binning (count + scatter passes, both kinds): the uvw read twice per
@@ -502,7 +576,7 @@ def main():
               HBM_PEAK_GBS),
         entry("backward columns + w-projection", "fftColsProjectKernel", ph["project"], "hbm",
               proj_bytes, HBM_PEAK_GBS),
-        entry("grid", "gridMfmaKernel + reducePartialsKernel", ph["grid"], "mfma", flops,
+        entry("grid", "gridMfmaF16Kernel + reducePartialsKernel", ph["grid"], "mfma", flops,
               FP32_PEAK_TFLOPS),
         entry("degrid", "degridTilesKernel", ph["degrid"], "fp32", flops, FP32_PEAK_TFLOPS),
         entry("binning", "binKernel (count, scatter) + scans", ph["bin"], "hbm", bin_bytes,
@@ -558,6 +632,15 @@ def main():
             accuracy = {"error": f"{type(e).__name__}: {e}"}
         log(f"accuracy report in {time.perf_counter() - t:.1f} s: {accuracy}")
 
+    plugin = None
+    if not args.no_plugin:
+        t = time.perf_counter()
+        try:
+            plugin = plugin_leg(w)
+        except Exception as e:  # reported, never hidden
+            plugin = {"error": f"{type(e).__name__}: {e}"}
+        log(f"plugin leg in {time.perf_counter() - t:.1f} s: {plugin}")
+
     facets = None
     if not args.no_facets:
         del w.vis_out
@@ -579,7 +662,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "visibilities/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 (fp64 positions/phases, fp16-pair kernel tables)"
+        "dtype": "fp32 (fp64 positions/phases; gridding products on fp16 matrix cores as "
+                 "hi/lo-split operands with fp32 sums)"
         if args.precision == "mixed" else ("fp32 (fp64 positions/phases)"
                                            if args.precision == "reference" else "fp32"),
         "data": "synthetic: seeded uvw of the 62-station 40 km log-spiral array, seeded "
@@ -601,6 +685,7 @@ def main():
                         "unit": "visibilities/s",
                         "what": "invert + predict with the binning plan reused (uvw are fixed "
                                 "across major cycles); value above includes the plan"},
+        "plugin_path": plugin,
         "plain_form": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain and "value" in plain else None,
         "rms_err_vs_fp64_dft": (accuracy.get(f"{args.precision}_os63", {}).get("image_rms_rel")
