@@ -1482,21 +1482,23 @@ gridMfmaKernel(GridArgs a, const uint32_t* __restrict__ binStart,
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
-// Scale of the low parts of split operands: x = h + l 2^-12, with l the FP16
-// rounding of (x - h) 2^12, which keeps l in FP16's normal range wherever h
-// is (the rounding error of h is up to 2^-11 |x|).
-constexpr float kLoScale = 4096.0f, kLoUnscale = 1.0f / 4096.0f;
-// Ranges of the split operands: |A| < 2^14, |B| < 2^13, so that both parts
-// stay below FP16's maximum (65504) and only operands below 2^-28 of the
-// largest fall into the subnormal range.
+// Ranges of the split operands x = h + l (h the FP16 rounding of x, l that
+// of x - h): |A| < 2^14, |B| < 2^13, so that both parts stay below FP16's
+// maximum (65504), and a low part lost to FP16's subnormal range (|l| <
+// 2^-14, |x| below about 2^-3) is below 2^-27 of the largest operand.
 constexpr int kAExp = 14, kBExp = 13;
 
+// x with the 13 low significand bits cleared (truncated to FP16 precision).
+__device__ __forceinline__ float truncToHalfBits(float x) {
+  return __uint_as_float(__float_as_uint(x) & 0xFFFFE000u);
+}
+
 // The mixed precision's tile product on the FP16 matrix cores: every FP32
-// operand x is split into its FP16 rounding h and the scaled FP16 rounding l
-// of its remainder (x = h + l 2^-12 to about 22 bits), and
-// G = Ah Bh + (Ah Bl + Al Bh) 2^-12 (the dropped Al Bl 2^-24 is below 2^-22
-// of each product) is accumulated in FP32 by v_mfma_f32_16x16x32_f16, 32
-// records per instruction, the high and the cross products in separate sums.
+// operand x is split into a high part h with FP16's precision (B: rounded,
+// once per table; A: truncated, cheaper per record) and the FP16 rounding l
+// of its remainder (x = h + l to 21-22 bits), and G = Ah Bh + Ah Bl + Al Bh
+// (the dropped Al Bl is below 2^-20 of each product) is accumulated in FP32
+// by v_mfma_f32_16x16x32_f16, 32 records per instruction.
 // B = the xKernel values times 2^(kBExp - eB), with 2^eB above the table's
 // largest value; A = v * yKernel, formed in FP32, scaled and split. Each
 // image of a 64-record batch is scaled by a power of two that brings its
@@ -1530,7 +1532,7 @@ gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
     if (c >= 0 && c < K) v = ktab[row * K + c];
     const float vs = v * bScale;
     const _Float16 h = (_Float16)vs;
-    const _Float16 l = (_Float16)((vs - (float)h) * kLoScale);
+    const _Float16 l = (_Float16)(vs - (float)h);
     kt[i] = v;
     kp[i] = uint32_t(__builtin_bit_cast(uint16_t, h)) |
             (uint32_t(__builtin_bit_cast(uint16_t, l)) << 16);
@@ -1607,12 +1609,9 @@ gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
 #pragma unroll
       for (int gi = 0; gi < NIMG; ++gi) myVis[lane * NIMG + gi] = v[gi];
       waveSync();
-      floatx4 d[2 * NIMG], dx[2 * NIMG];  // high products, cross products
+      floatx4 d[2 * NIMG];
 #pragma unroll
-      for (int c = 0; c < 2 * NIMG; ++c) {
-        d[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        dx[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-      }
+      for (int c = 0; c < 2 * NIMG; ++c) d[c] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int G = 0; G < 2; ++G) {
         int ps[8];
@@ -1639,35 +1638,32 @@ gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
 #pragma unroll
           for (int t = 0; t < 8; ++t) {
             const float2 vv = myVis[(32 * G + 8 * lg + t) * NIMG + gi];
-            float xr = vv.x * kyv[t], xi = vv.y * kyv[t];
-            // the FP32 products as values: the compiler otherwise rounds the
-            // exact product to FP16 (v_fma_mixlo_f16) for one use of h and
-            // the FP32 product for the other, and where those differ (near
-            // FP16 ties) the split loses an FP16 unit
-            __asm__("" : "+v"(xr), "+v"(xi));
-            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
-            ah[0][t] = hr;
-            ah[1][t] = hi;
-            // (x - h) is exact in FP32, and so is its scaling
-            al[0][t] = (_Float16)((xr - (float)hr) * kLoScale);
-            al[1][t] = (_Float16)((xi - (float)hi) * kLoScale);
+            const float xr = vv.x * kyv[t], xi = vv.y * kyv[t];
+            // high part: x with its significand cut to FP16's 11 bits (exact
+            // in FP16 for |x| >= 2^-14; below, its FP16 rounding differs by
+            // less than 2^-24, under 2^-38 of the largest operand); low part:
+            // the exact remainder, rounded to FP16 (up to 2^-21 |x|)
+            const float hr = truncToHalfBits(xr), hi = truncToHalfBits(xi);
+            ah[0][t] = (_Float16)hr;
+            ah[1][t] = (_Float16)hi;
+            al[0][t] = (_Float16)(xr - hr);
+            al[1][t] = (_Float16)(xi - hi);
           }
 #pragma unroll
           for (int c = 0; c < 2; ++c) {
-            d[2 * gi + c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bh, d[2 * gi + c],
-                                                                   0, 0, 0);
-            dx[2 * gi + c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bl, dx[2 * gi + c],
-                                                                    0, 0, 0);
-            dx[2 * gi + c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[c], bh, dx[2 * gi + c],
-                                                                    0, 0, 0);
+            // the small cross products first, then the high one
+            floatx4& dd = d[2 * gi + c];
+            dd = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bl, dd, 0, 0, 0);
+            dd = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[c], bh, dd, 0, 0, 0);
+            dd = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bh, dd, 0, 0, 0);
           }
         }
       }
 #pragma unroll
       for (int c = 0; c < 2 * NIMG; ++c) {
-        const float u = up[c / 2], ux = u * kLoUnscale;
+        const float u = up[c / 2];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[c][r] = acc[c][r] + (d[c][r] * u + dx[c][r] * ux);
+        for (int r = 0; r < 4; ++r) acc[c][r] = acc[c][r] + d[c][r] * u;
       }
       waveSync();  // the staging is rewritten next batch
     }
