@@ -412,6 +412,8 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
     g->noFusedProject = f && f[0] == '1';
     const char* h = std::getenv("HWS_NO_F16_MFMA");
     g->noF16Mfma = h && h[0] == '1';
+    const char* d = std::getenv("HWS_DEGRID_BLOCKS_PER_CU");
+    if (d && std::atoi(d) > 0 && std::atoi(d) <= 16) g->degridBlocksPerCu = std::atoi(d);
   }
   // uv tile edge of the binned forms: divides W and H, a multiple of 4, at
   // least K - 1 (a footprint then spans at most two tiles per axis) and with

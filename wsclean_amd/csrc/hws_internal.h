@@ -404,7 +404,8 @@ struct hws_gridder {
   // grid with the VALU cell kernel at T = 16 instead of the matrix-core one
   // (HWS_NO_MFMA=1 in the environment; for comparisons)
   bool noMfma = false;
-  bool noF16Mfma = false;  // the mixed precision grids with the FP32 matrix-core kernel
+  bool noF16Mfma = false;
+  int degridBlocksPerCu = 5;  // the mixed precision grids with the FP32 matrix-core kernel
   // run the backward columns pass and the projection apart, and the forward
   // rows pass four layers per workgroup (the forms the fused kernels replace)
   bool noFusedProject = false;

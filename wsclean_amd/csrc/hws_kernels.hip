@@ -1799,6 +1799,78 @@ __device__ __forceinline__ void degridRecord(const GridArgs& a, uint2 rec,
 }
 
 constexpr int kDegridQueue = 128;  // staged primary records per wave
+// The (layer, tile) item's subgrid position.
+struct DegridItem {
+  WorkItem wi;
+  int layer, x0, y0;
+};
+
+__device__ __forceinline__ DegridItem degridItem(const GridArgs& a, uint32_t item,
+                                                 uint32_t binLo, uint32_t binHi,
+                                                 const uint32_t* __restrict__ binStart,
+                                                 const uint32_t* __restrict__ itemStart,
+                                                 const uint32_t* __restrict__ itemBin) {
+  DegridItem d;
+  d.wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
+  d.layer = int(d.wi.bin / uint32_t(a.tilesPerLayer));
+  const int rem = int(d.wi.bin - uint32_t(d.layer) * uint32_t(a.tilesPerLayer));
+  const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
+  d.x0 = tx * a.ga.tileSize;
+  d.y0 = ty * a.ga.tileSize;
+  return d;
+}
+
+// Layer value of subgrid cell `cell` of item d, image slot gi.
+__device__ __forceinline__ float2 subgridCell(const GridArgs& a, const DegridItem& d, int cell,
+                                              int gi, const float2* __restrict__ layers) {
+  const int sg = a.sg;
+  const int cy = cell / sg, cx = cell - cy * sg;
+  const int gx = (d.x0 + cx) % a.ga.W, gy = (d.y0 + cy) % a.ga.H;
+  const size_t HW = size_t(a.ga.W) * a.ga.H;
+  return layers[(size_t(a.imgs.idx[gi]) * a.nLayersPass + (d.layer - a.passStart)) * HW +
+                cellIndex(gx, gy, a.ga.W, a.ga.blocked)];
+}
+
+// The primary records of one item, degridded from its subgrid `sub` in LDS:
+// wave w reads records w * 64 + lane + k * blockDim; qn (wave-uniform)
+// primary records wait in the wave's queue, at most 63 between batches.
+template <int NIMG, int KK>
+__device__ __forceinline__ void degridItemRecords(const GridArgs& a, const WorkItem& wi,
+                                                  const uint2* __restrict__ recs,
+                                                  const float* kt, const float2* sub,
+                                                  uint2* queue, const double* __restrict__ uvw,
+                                                  const double* __restrict__ wl,
+                                                  float2* __restrict__ out) {
+  const int lane = int(threadIdx.x & 63u);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t start = wi.start, count = wi.count;
+  int qn = 0;
+  for (uint32_t b = threadIdx.x & ~63u; b < count; b += blockDim.x) {
+    const uint32_t it = b + uint32_t(lane);
+    uint2 rec = make_uint2(0u, 0u);
+    bool primary = false;
+    if (it < count) {
+      rec = recs[start + it];
+      primary = ((rec.x >> 30) & 1u) != 0u;
+    }
+    const unsigned long long mask = __ballot(primary);
+    if (primary) queue[qn + __popcll(mask & below)] = rec;
+    qn += __popcll(mask);
+    if (qn >= 64) {
+      waveSync();
+      const uint2 mine = queue[lane];
+      const int left = qn - 64;
+      uint2 tail = make_uint2(0u, 0u);
+      if (lane < left) tail = queue[64 + lane];
+      waveSync();
+      if (lane < left) queue[lane] = tail;
+      qn = left;
+      degridRecord<NIMG, KK>(a, mine, kt, sub, a.sg, uvw, wl, out);
+    }
+  }
+  waveSync();
+  if (lane < qn) degridRecord<NIMG, KK>(a, queue[lane], kt, sub, a.sg, uvw, wl, out);
+}
 
 // Degridding of one image kind, one work item per workgroup at a time: the
 // (T + K - 1)^2 subgrid around the item's (layer, tile) is gathered from the
@@ -1816,66 +1888,25 @@ degridTilesKernel(GridArgs a, const uint32_t* __restrict__ binStart,
                   const double* __restrict__ wl, const float2* __restrict__ layers,
                   float2* __restrict__ out) {
   extern __shared__ float smem[];
-  const int K = KK ? KK : a.ga.K, OS = a.ga.OS, sg = a.sg, sg2 = a.sg * a.sg;
+  const int K = KK ? KK : a.ga.K, OS = a.ga.OS, sg2 = a.sg * a.sg;
   float* kt = smem;
   float2* sub = reinterpret_cast<float2*>(smem + ((OS * K + 3) & ~3));  // [sg2][NIMG]
-  uint2* queue =
-      reinterpret_cast<uint2*>(sub + size_t(NIMG) * sg2) + (threadIdx.x >> 6) * kDegridQueue;
-  const int lane = int(threadIdx.x & 63u);
-  const unsigned long long below = (1ull << lane) - 1ull;
+  uint2* queue = reinterpret_cast<uint2*>(sub + size_t(NIMG) * sg2) +
+                 (threadIdx.x >> 6) * kDegridQueue;
   loadKernelTable(kt, ktab, khalf, OS * K);
   const uint32_t binLo = uint32_t(a.layerLo) * uint32_t(a.tilesPerLayer);
   const uint32_t binHi = uint32_t(a.layerHi) * uint32_t(a.tilesPerLayer);
-  const size_t HW = size_t(a.ga.W) * a.ga.H;
   const uint32_t nItems = passItems(binLo, binHi, itemStart);
   for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
     // every lane decodes the item (uniform loads)
-    const WorkItem wi = decodeItem(item, binLo, binHi, binStart, itemStart, a.chunk, itemBin);
-    const uint32_t bin = wi.bin, start = wi.start, count = wi.count;
-    if (count == 0) continue;  // block-uniform
-    const int layer = int(bin / uint32_t(a.tilesPerLayer));
-    const int rem = int(bin - uint32_t(layer) * uint32_t(a.tilesPerLayer));
-    const int ty = rem / a.ga.tilesX, tx = rem - ty * a.ga.tilesX;
-    const int x0 = tx * a.ga.tileSize, y0 = ty * a.ga.tileSize;
+    const DegridItem d = degridItem(a, item, binLo, binHi, binStart, itemStart, itemBin);
+    if (d.wi.count == 0) continue;  // block-uniform
     __syncthreads();  // previous item's readers are done with sub and the queues
-    for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x) {
-      const int cy = cell / sg, cx = cell - cy * sg;
-      const int gx = (x0 + cx) % a.ga.W, gy = (y0 + cy) % a.ga.H;
+    for (int cell = threadIdx.x; cell < sg2; cell += blockDim.x)
 #pragma unroll
-      for (int gi = 0; gi < NIMG; ++gi)
-        sub[cell * NIMG + gi] =
-            layers[(size_t(a.imgs.idx[gi]) * a.nLayersPass + (layer - a.passStart)) * HW +
-                   cellIndex(gx, gy, a.ga.W, a.ga.blocked)];
-    }
+      for (int gi = 0; gi < NIMG; ++gi) sub[cell * NIMG + gi] = subgridCell(a, d, cell, gi, layers);
     __syncthreads();
-    // wave w reads records w * 64 + lane + k * blockDim; qn (wave-uniform)
-    // primary records wait in the queue, at most 63 between batches
-    int qn = 0;
-    for (uint32_t b = threadIdx.x & ~63u; b < count; b += blockDim.x) {
-      const uint32_t it = b + uint32_t(lane);
-      uint2 rec = make_uint2(0u, 0u);
-      bool primary = false;
-      if (it < count) {
-        rec = recs[start + it];
-        primary = ((rec.x >> 30) & 1u) != 0u;
-      }
-      const unsigned long long mask = __ballot(primary);
-      if (primary) queue[qn + __popcll(mask & below)] = rec;
-      qn += __popcll(mask);
-      if (qn >= 64) {
-        waveSync();
-        const uint2 mine = queue[lane];
-        const int left = qn - 64;
-        uint2 tail = make_uint2(0u, 0u);
-        if (lane < left) tail = queue[64 + lane];
-        waveSync();
-        if (lane < left) queue[lane] = tail;
-        qn = left;
-        degridRecord<NIMG, KK>(a, mine, kt, sub, sg, uvw, wl, out);
-      }
-    }
-    waveSync();
-    if (lane < qn) degridRecord<NIMG, KK>(a, queue[lane], kt, sub, sg, uvw, wl, out);
+    degridItemRecords<NIMG, KK>(a, d.wi, recs, kt, sub, queue, uvw, wl, out);
   }
 }
 
@@ -2967,7 +2998,7 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
   // five waves per SIMD at two images): a static share of items per
   // workgroup, and no second round of workgroups running at low occupancy
   // (measured: 8 per CU, 13.2 ms per step at the 4096^2 benchmark; 5, 12.2)
-  const int blocks = persistentBlocks(g, 5);
+  const int blocks = persistentBlocks(g, g->degridBlocksPerCu);
   const __half2* khalf = g->cfg.precision == HWS_PRECISION_MIXED ? g->dKernelPair : nullptr;
   const size_t ldsMax = maxLdsBytes(g);
   float2* out = reinterpret_cast<float2*>(dVisOut);
@@ -2987,24 +3018,23 @@ hws_status degridBinned(hws_gridder* g, float* dVisOut, int pass) {
         a.imgs.idx[i] = (i0 + i < kp.nImg) ? kp.img[i0 + i] : 0;
       const size_t lds = gridLdsBytes(g, n, true);
       switch (n) {
+#define HWS_DEGRID_GO(N, KK)                                                                  \
+  hipLaunchKernelGGL((degridTilesKernel<N, KK>), dim3(blocks), dim3(kThreads), lds, g->stream,   \
+                     a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted, g->dKernel, khalf,  \
+                     pl.uvw, pl.wavelengths, reinterpret_cast<const float2*>(g->dLayers), out)
 #define HWS_DEGRID_CASE(N)                                                                \
   case N:                                                                                   \
     if (g->K == 7)                                                                          \
-      hipLaunchKernelGGL((degridTilesKernel<N, 7>), dim3(blocks), dim3(kThreads), lds,        \
-                         g->stream, a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted,   \
-                         g->dKernel, khalf, pl.uvw, pl.wavelengths,                         \
-                         reinterpret_cast<const float2*>(g->dLayers), out);                 \
+      HWS_DEGRID_GO(N, 7);                                                                  \
     else                                                                                    \
-      hipLaunchKernelGGL((degridTilesKernel<N, 0>), dim3(blocks), dim3(kThreads), lds,        \
-                         g->stream, a, kp.binStart, kp.itemStart, kp.itemBin, kp.recSorted,   \
-                         g->dKernel, khalf, pl.uvw, pl.wavelengths,                         \
-                         reinterpret_cast<const float2*>(g->dLayers), out);                 \
+      HWS_DEGRID_GO(N, 0);                                                                  \
     break;
         HWS_DEGRID_CASE(1)
         HWS_DEGRID_CASE(2)
         HWS_DEGRID_CASE(3)
         HWS_DEGRID_CASE(4)
 #undef HWS_DEGRID_CASE
+#undef HWS_DEGRID_GO
       }
       HWS_TRY(checkHip(g, hipGetLastError(), "degrid launch"));
     }
