@@ -173,14 +173,35 @@ def _image_error(W, H, dl, dm, uvw, one, vis, n, n_w, w_lo, w_hi, dft_img, cutou
     return _rel(im, dft_img)[0]
 
 
+def padded_size(width, padding, multiple=16):
+    """The smallest gridded size >= width * padding that is a multiple of
+    `multiple` (the uv tile) and 7-smooth (the transforms' radices), as the
+    reference pads an image before trimming it (WSClean's -padding)."""
+    n = int(math.ceil(width * padding / multiple - 1e-9)) * multiple
+    while True:
+        m = n
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return n
+        n += multiple
+
+
 def sweep(device="cuda:0", n_ws=(32, 48, 64, 96, 128, 192, 256), oversamplings=(63, 1023),
-          precision="mixed", n_samples=100_000, seed=303):
+          precision="mixed", n_samples=100_000, seed=303, paddings=(1.0,),
+          padded_n_ws=(32, 64, 128)):
     """Image and visibility errors of the precision protocol (the report's
     workload) against the FP64 direct transform, for every w-layer count in
     n_ws and every kernel oversampling: where the w-stacking approximation
     (each sample on its nearest layer) stops dominating the error, and what
-    the extra layers cost. The reference's suggestion for this workload is
-    reported beside it (suggested_w_layers in autotune)."""
+    the extra layers cost. With paddings above 1 the image is also gridded
+    at padded_size(1024, padding) with the same pixels and the same central
+    cutout compared (the model embedded at the centre for the predict), for
+    the layer counts in padded_n_ws: the kernel's aliasing at the image edge,
+    which more layers do not remove, falls with the padding. The reference's
+    suggestion for this workload is reported beside it (suggested_w_layers
+    in autotune)."""
     import torch
     from .autotune import suggested_w_layers
     from .gridder import DeviceGridder
@@ -197,21 +218,28 @@ def sweep(device="cuda:0", n_ws=(32, 48, 64, 96, 128, 192, 256), oversamplings=(
     w_lo, w_hi = float(aw.min()), float(aw.max())
     dft_img, dft_vis = direct_reference(wk, device)
     rows = []
-    for os_ in oversamplings:
-        for n_w in n_ws:
-            g = DeviceGridder(W, W, dl, dl, 7, os_, precision=precision, n_images=1)
-            g.prepare_w_layers(n_w, w_lo, w_hi)
-            img = torch.zeros((1, W, W), dtype=torch.float32, device=dev)
-            g.plan(uvw, one)
-            g.invert(vis, 1.0 / n, img)
-            pred = torch.empty_like(vis)
-            g.predict(model.reshape(1, W, W), None, pred)
-            g.synchronize()
-            g.close()
-            i_rms, v_rms = errors(img.cpu().numpy()[0], pred.cpu().numpy(), dft_img, dft_vis,
-                                  cut)
-            rows.append({"n_w_layers": n_w, "oversampling": os_, "image_rms_rel": i_rms,
-                         "vis_rms_rel": v_rms})
+    for padding in paddings:
+        Wp = W if padding <= 1.0 else padded_size(W, padding)
+        model_p = model
+        if Wp != W:
+            model_p = torch.zeros((Wp, Wp), dtype=torch.float32, device=dev)
+            o = (Wp - W) // 2
+            model_p[o:o + W, o:o + W] = model
+        for os_ in oversamplings:
+            for n_w in (n_ws if Wp == W else padded_n_ws):
+                g = DeviceGridder(Wp, Wp, dl, dl, 7, os_, precision=precision, n_images=1)
+                g.prepare_w_layers(n_w, w_lo, w_hi)
+                img = torch.zeros((1, Wp, Wp), dtype=torch.float32, device=dev)
+                g.plan(uvw, one)
+                g.invert(vis, 1.0 / n, img)
+                pred = torch.empty_like(vis)
+                g.predict(model_p.reshape(1, Wp, Wp), None, pred)
+                g.synchronize()
+                g.close()
+                i_rms, v_rms = errors(img.cpu().numpy()[0], pred.cpu().numpy(), dft_img,
+                                      dft_vis, cut)
+                rows.append({"n_w_layers": n_w, "oversampling": os_, "padding": Wp / W,
+                             "gridded_size": Wp, "image_rms_rel": i_rms, "vis_rms_rel": v_rms})
     return {"precision": precision,
             "suggested_w_layers": suggested_w_layers(W, W, dl, dl, w_lo, w_hi),
             "w_range": (w_lo, w_hi), "rows": rows}
