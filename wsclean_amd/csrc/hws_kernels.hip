@@ -1488,9 +1488,19 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 // 2^-14, |x| below about 2^-3) is below 2^-27 of the largest operand.
 constexpr int kAExp = 14, kBExp = 13;
 
-// x with the 13 low significand bits cleared (truncated to FP16 precision).
-__device__ __forceinline__ float truncToHalfBits(float x) {
-  return __uint_as_float(__float_as_uint(x) & 0xFFFFE000u);
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// The FP16 roundings of x0 - h0 and x1 - h1 packed into one word, h0 and h1
+// the two halves of h (x - h is exact in FP32 where h is x rounded toward
+// zero; v_fma_mix forms it from the FP16 operand and rounds once). Written
+// by hand: the compiler does not select v_fma_mix for it.
+__device__ __forceinline__ uint32_t lowParts(float x0, float x1, uint32_t h) {
+  uint32_t lo;
+  __asm__("v_fma_mixlo_f16 %0, %1, 1.0, -%3 op_sel_hi:[0,0,1]\n\t"
+          "v_fma_mixhi_f16 %0, %2, 1.0, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+          : "=&v"(lo)
+          : "v"(x0), "v"(x1), "v"(h));
+  return lo;
 }
 
 // The mixed precision's tile product on the FP16 matrix cores: every FP32
@@ -1634,21 +1644,26 @@ gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
         }
 #pragma unroll
         for (int gi = 0; gi < NIMG; ++gi) {
-          half8 ah[2], al[2];
+          // per pair of records: the high parts, x rounded toward zero to
+          // FP16 (its significand cut to 11 bits), packed by one
+          // v_cvt_pkrtz; the low parts, x - h rounded to FP16 (up to
+          // 2^-21 |x|), by one v_fma_mix each, written into the two halves
+          u32x4 hw[2], lw[2];
 #pragma unroll
-          for (int t = 0; t < 8; ++t) {
-            const float2 vv = myVis[(32 * G + 8 * lg + t) * NIMG + gi];
-            const float xr = vv.x * kyv[t], xi = vv.y * kyv[t];
-            // high part: x with its significand cut to FP16's 11 bits (exact
-            // in FP16 for |x| >= 2^-14; below, its FP16 rounding differs by
-            // less than 2^-24, under 2^-38 of the largest operand); low part:
-            // the exact remainder, rounded to FP16 (up to 2^-21 |x|)
-            const float hr = truncToHalfBits(xr), hi = truncToHalfBits(xi);
-            ah[0][t] = (_Float16)hr;
-            ah[1][t] = (_Float16)hi;
-            al[0][t] = (_Float16)(xr - hr);
-            al[1][t] = (_Float16)(xi - hi);
+          for (int t = 0; t < 8; t += 2) {
+            const float2 v0 = myVis[(32 * G + 8 * lg + t) * NIMG + gi];
+            const float2 v1 = myVis[(32 * G + 8 * lg + t + 1) * NIMG + gi];
+            const float xr0 = v0.x * kyv[t], xr1 = v1.x * kyv[t + 1];
+            const float xi0 = v0.y * kyv[t], xi1 = v1.y * kyv[t + 1];
+            const uint32_t hR = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(xr0, xr1));
+            const uint32_t hI = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(xi0, xi1));
+            hw[0][t / 2] = hR;
+            hw[1][t / 2] = hI;
+            lw[0][t / 2] = lowParts(xr0, xr1, hR);
+            lw[1][t / 2] = lowParts(xi0, xi1, hI);
           }
+          const half8 ah[2] = {__builtin_bit_cast(half8, hw[0]), __builtin_bit_cast(half8, hw[1])};
+          const half8 al[2] = {__builtin_bit_cast(half8, lw[0]), __builtin_bit_cast(half8, lw[1])};
 #pragma unroll
           for (int c = 0; c < 2; ++c) {
             // the small cross products first, then the high one
