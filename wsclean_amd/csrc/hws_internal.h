@@ -103,34 +103,14 @@ __host__ __device__ inline double nPrimeSeries(double l, double m) {
 
 #ifdef __HIPCC__
 // sin and cos of 2 pi f for a fraction of a turn f in [-0.5, 0.5]: the
-// quarter turn k = rint(4 f) and the remainder r = 4 f - k in [-0.5, 0.5]
-// (both exact); sin and cos of (pi / 2) r from their Taylor series to r^9
-// and r^10 (truncation below 2e-9), rotated by k quarter turns. Within
-// 1e-7 of the exact values; about half the instructions of sincospif, whose
-// general range reduction this argument does not need.
+// hardware's v_sin_f32 and v_cos_f32, which take their argument in turns.
+// Measured on 4M arguments spread over [-0.5, 0.5]: within 1.25e-7 of the
+// exact values (RMS 3.5e-8), against 1e-7 for a Taylor evaluation after a
+// quarter-turn reduction that costs 17 full-rate instructions where these
+// are two quarter-rate ones.
 __device__ __forceinline__ void turnSinCos(float f, float& s, float& c) {
-#pragma clang fp contract(off)
-  const float t = 4.0f * f;
-  const float k = rintf(t);
-  const float r = t - k;
-  const float r2 = r * r;
-  float ps = 1.60441185e-4f;
-  ps = fmaf(ps, r2, -4.68175413e-3f);
-  ps = fmaf(ps, r2, 7.96926262e-2f);
-  ps = fmaf(ps, r2, -6.45964098e-1f);
-  ps = fmaf(ps, r2, 1.57079633f);
-  const float sa = ps * r;
-  float pc = -2.52020424e-5f;
-  pc = fmaf(pc, r2, 9.19260275e-4f);
-  pc = fmaf(pc, r2, -2.08634808e-2f);
-  pc = fmaf(pc, r2, 2.53669508e-1f);
-  pc = fmaf(pc, r2, -1.23370055f);
-  const float ca = fmaf(pc, r2, 1.0f);
-  const int q = int(k) & 3;
-  const float s0 = (q & 1) ? ca : sa;
-  const float c0 = (q & 1) ? sa : ca;
-  s = (q & 2) ? -s0 : s0;
-  c = ((q + 1) & 2) ? -c0 : c0;
+  s = __builtin_amdgcn_sinf(f);
+  c = __builtin_amdgcn_cosf(f);
 }
 
 __device__ __forceinline__ void kahanAdd(float& sum, float& comp, float term) {
