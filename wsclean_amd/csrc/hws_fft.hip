@@ -171,9 +171,11 @@ __device__ __forceinline__ void fftLds(float2* buf, int pitch, int nfft, int n,
                        int nthr) {
   int ns = 1;
   while (radices != 0ull) {
-    const int R = int(radices & 15ull);
+    const int code = int(radices & 15ull);
+    const int R = code == 1 ? 16 : code;  // (radix 16 is packed as 1)
     radices >>= 4;
     switch (R) {
+      case 16: stockhamStage<SIGN, 16>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
       case 8: stockhamStage<SIGN, 8>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
       case 4: stockhamStage<SIGN, 4>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
       case 2: stockhamStage<SIGN, 2>(buf, n, pitch, nfft, ns, tw, tid, nthr); break;
@@ -919,8 +921,8 @@ fftColsPow2Kernel(float2* __restrict__ data, int W, int nColBlocks,
 
 // rows: grid (ceil(H / rowsPerBlock), nbatch); each block transforms
 // rowsPerBlock consecutive rows of one layer.
-template <int SIGN>
-__global__ void __launch_bounds__(1024)
+template <int SIGN, int NT>
+__global__ void __launch_bounds__(NT)
 fftRowsKernel(float2* __restrict__ data, int W, int H, int rowsPerBlock,
               unsigned long long radices, const float2* __restrict__ tw) {
   extern __shared__ float2 lds[];
@@ -938,8 +940,8 @@ fftRowsKernel(float2* __restrict__ data, int W, int H, int rowsPerBlock,
 
 // columns: grid (nColBlocks, nbatch); block handles columns
 // [c0, c0 + bc) of one layer; LDS holds [bc][H + 1].
-template <int SIGN>
-__global__ void __launch_bounds__(1024)
+template <int SIGN, int NT>
+__global__ void __launch_bounds__(NT)
 fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
               unsigned long long radices, const float2* __restrict__ tw) {
   extern __shared__ float2 lds[];
@@ -986,8 +988,8 @@ __device__ __forceinline__ bool bitAt(const uint32_t* bits, size_t b) {
   return (bits[b >> 5] >> (b & 31)) & 1u;
 }
 
-template <int SIGN, int MODE, bool REF>
-__global__ void __launch_bounds__(1024)
+template <int SIGN, int MODE, bool REF, int NT>
+__global__ void __launch_bounds__(NT)
 fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
                             unsigned long long radices, const float2* __restrict__ tw,
                             GenericSparseArgs sp, ModelSource ms) {
@@ -1020,8 +1022,8 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
   for (int i = tid; i < 4 * W; i += nthr) base[i] = lds[(i & 3) * pitch + (i >> 2)];
 }
 
-template <int SIGN, int MODE>
-__global__ void __launch_bounds__(1024)
+template <int SIGN, int MODE, int NT>
+__global__ void __launch_bounds__(NT)
 fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlocks,
                             unsigned long long radices, const float2* __restrict__ tw,
                             GenericSparseArgs sp) {
@@ -1062,9 +1064,72 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
   }
 }
 
+// The runtime-radix kernels are compiled for three launch bounds; a launch
+// takes the smallest that holds its threads, so the compiler may give each
+// thread the registers its stages need (at 1024 threads it holds 128 and
+// spills).
+template <int SIGN, int MODE, bool REF>
+void launchRowsGeneric(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
+                       int H, unsigned long long rad, const float2* tw, GenericSparseArgs sp,
+                       ModelSource ms) {
+  if (threads <= 256)
+    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 256>), grid,
+                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms);
+  else if (threads <= 512)
+    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 512>), grid,
+                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms);
+  else
+    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 1024>), grid,
+                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms);
+}
+
+template <int SIGN, int MODE>
+void launchColsGeneric(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
+                       int H, int nColBlocks, unsigned long long rad, const float2* tw,
+                       GenericSparseArgs sp) {
+  if (threads <= 256)
+    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 256>), grid, dim3(threads), lds,
+                       st, data, W, H, nColBlocks, rad, tw, sp);
+  else if (threads <= 512)
+    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 512>), grid, dim3(threads), lds,
+                       st, data, W, H, nColBlocks, rad, tw, sp);
+  else
+    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 1024>), grid, dim3(threads), lds,
+                       st, data, W, H, nColBlocks, rad, tw, sp);
+}
+
+template <int SIGN>
+void launchRowsPlain(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
+                     int H, int rows, unsigned long long rad, const float2* tw) {
+  if (threads <= 256)
+    hipLaunchKernelGGL((fftRowsKernel<SIGN, 256>), grid, dim3(threads), lds, st, data, W, H, rows,
+                       rad, tw);
+  else if (threads <= 512)
+    hipLaunchKernelGGL((fftRowsKernel<SIGN, 512>), grid, dim3(threads), lds, st, data, W, H, rows,
+                       rad, tw);
+  else
+    hipLaunchKernelGGL((fftRowsKernel<SIGN, 1024>), grid, dim3(threads), lds, st, data, W, H,
+                       rows, rad, tw);
+}
+
+template <int SIGN>
+void launchColsPlain(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
+                     int H, int bc, int nColBlocks, unsigned long long rad, const float2* tw) {
+  if (threads <= 256)
+    hipLaunchKernelGGL((fftColsKernel<SIGN, 256>), grid, dim3(threads), lds, st, data, W, H, bc,
+                       nColBlocks, rad, tw);
+  else if (threads <= 512)
+    hipLaunchKernelGGL((fftColsKernel<SIGN, 512>), grid, dim3(threads), lds, st, data, W, H, bc,
+                       nColBlocks, rad, tw);
+  else
+    hipLaunchKernelGGL((fftColsKernel<SIGN, 1024>), grid, dim3(threads), lds, st, data, W, H, bc,
+                       nColBlocks, rad, tw);
+}
+
 unsigned long long packRadices(const FftPlan& plan) {
   unsigned long long r = 0;
-  for (int i = plan.nstages - 1; i >= 0; --i) r = (r << 4) | unsigned(plan.radix[i]);
+  for (int i = plan.nstages - 1; i >= 0; --i)
+    r = (r << 4) | unsigned(plan.radix[i] == 16 ? 1 : plan.radix[i]);
   return r;
 }
 
@@ -1089,7 +1154,9 @@ hws_status makeFftPlan(hws_gridder* g, int n, FftPlan& plan) {
   plan.n = n;
   plan.nstages = 0;
   int m = n;
-  // radix 8 first, then 4 / 2, then the odd factors
+  // radix 16 first, then 8 / 4 / 2, then the odd factors (fewer stages, each
+  // a round trip through LDS)
+  while (m % 16 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 16; m /= 16; }
   while (m % 8 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 8; m /= 8; }
   while (m % 4 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 4; m /= 4; }
   while (m % 2 == 0 && plan.nstages < kMaxFftStages) { plan.radix[plan.nstages++] = 2; m /= 2; }
@@ -1296,34 +1363,26 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     }
     if (model) msrc = *model;
     const dim3 gr(H / 4, nbatch), gc(nColBlocks, nbatch);
+    hipStream_t st = g->stream;
     if (sign > 0) {
       if (sparse) {
-        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<+1, 1, false>), gr, dim3(tr), ldsR,
-                           g->stream, data, W, H, radR, twRow, rs, msrc);
-        hipLaunchKernelGGL((fftColsBlockedGenericKernel<+1, 1>), gc, dim3(tc), ldsC, g->stream,
-                           data, W, H, nColBlocks, radC, twCol, cs);
+        launchRowsGeneric<+1, 1, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchColsGeneric<+1, 1>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       } else {
-        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<+1, 0, false>), gr, dim3(tr), ldsR,
-                           g->stream, data, W, H, radR, twRow, rs, msrc);
-        hipLaunchKernelGGL((fftColsBlockedGenericKernel<+1, 0>), gc, dim3(tc), ldsC, g->stream,
-                           data, W, H, nColBlocks, radC, twCol, cs);
+        launchRowsGeneric<+1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchColsGeneric<+1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       }
     } else {
       if (model && model->precision == HWS_PRECISION_REFERENCE)
-        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<-1, 2, true>), gr, dim3(tr), ldsR,
-                           g->stream, data, W, H, radR, twRow, rs, msrc);
+        launchRowsGeneric<-1, 2, true>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       else if (model)
-        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<-1, 2, false>), gr, dim3(tr), ldsR,
-                           g->stream, data, W, H, radR, twRow, rs, msrc);
+        launchRowsGeneric<-1, 2, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       else
-        hipLaunchKernelGGL((fftRowsBlockedGenericKernel<-1, 0, false>), gr, dim3(tr), ldsR,
-                           g->stream, data, W, H, radR, twRow, rs, msrc);
+        launchRowsGeneric<-1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       if (sparse)
-        hipLaunchKernelGGL((fftColsBlockedGenericKernel<-1, 2>), gc, dim3(tc), ldsC, g->stream,
-                           data, W, H, nColBlocks, radC, twCol, cs);
+        launchColsGeneric<-1, 2>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       else
-        hipLaunchKernelGGL((fftColsBlockedGenericKernel<-1, 0>), gc, dim3(tc), ldsC, g->stream,
-                           data, W, H, nColBlocks, radC, twCol, cs);
+        launchColsGeneric<-1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
     }
     return checkHip(g, hipGetLastError(), "fft blocked generic launch");
   }
@@ -1485,9 +1544,9 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     const unsigned long long radices = packRadices(g->rowPlan);
     const float2* tw = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
     if (sign > 0)
-      hipLaunchKernelGGL(fftRowsKernel<+1>, grid, dim3(threads), lds, g->stream, data, W, H, rows, radices, tw);
+      launchRowsPlain<+1>(threads, grid, lds, g->stream, data, W, H, rows, radices, tw);
     else
-      hipLaunchKernelGGL(fftRowsKernel<-1>, grid, dim3(threads), lds, g->stream, data, W, H, rows, radices, tw);
+      launchRowsPlain<-1>(threads, grid, lds, g->stream, data, W, H, rows, radices, tw);
     hws_status st = checkHip(g, hipGetLastError(), "fft rows launch");
     if (st != HWS_OK) return st;
   }
@@ -1506,9 +1565,9 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     const unsigned long long radices = packRadices(g->colPlan);
     const float2* tw = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
     if (sign > 0)
-      hipLaunchKernelGGL(fftColsKernel<+1>, grid, dim3(threads), lds, g->stream, data, W, H, bc, nColBlocks, radices, tw);
+      launchColsPlain<+1>(threads, grid, lds, g->stream, data, W, H, bc, nColBlocks, radices, tw);
     else
-      hipLaunchKernelGGL(fftColsKernel<-1>, grid, dim3(threads), lds, g->stream, data, W, H, bc, nColBlocks, radices, tw);
+      launchColsPlain<-1>(threads, grid, lds, g->stream, data, W, H, bc, nColBlocks, radices, tw);
     hws_status st = checkHip(g, hipGetLastError(), "fft cols launch");
     if (st != HWS_OK) return st;
   }
