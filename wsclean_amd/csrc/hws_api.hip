@@ -398,7 +398,6 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   if ((st = checkHip(g, hipMemcpy(g->dCorrXf, cx.data(), cx.size() * sizeof(float), hipMemcpyHostToDevice), "correction upload")) != HWS_OK) return bail(st);
   if ((st = checkHip(g, hipMemcpy(g->dCorrYf, cy.data(), cy.size() * sizeof(float), hipMemcpyHostToDevice), "correction upload")) != HWS_OK) return bail(st);
 
-  // blocked layer layout where the coalesced radix-16 transforms cover the size
   // blocked layer layout: always at the power-of-two sizes with compile-time
   // transforms; at other sizes for the binned forms only (the plain form's
   // per-tap global atomics touch fewer cache lines in row-major order)
@@ -412,6 +411,8 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
     g->noFusedProject = f && f[0] == '1';
     const char* h = std::getenv("HWS_NO_F16_MFMA");
     g->noF16Mfma = h && h[0] == '1';
+    const char* tl = std::getenv("HWS_FFT_TWO_LINES");
+    g->forceTwoLines = tl && tl[0] == '1';
     const char* d = std::getenv("HWS_DEGRID_BLOCKS_PER_CU");
     if (d && std::atoi(d) > 0 && std::atoi(d) <= 16) g->degridBlocksPerCu = std::atoi(d);
   }
@@ -440,6 +441,8 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   }
   if ((st = makeFftPlan(g, g->W, g->rowPlan)) != HWS_OK) return bail(st);
   if ((st = makeFftPlan(g, g->H, g->colPlan)) != HWS_OK) return bail(st);
+  if (g->blocked && !(fastBlockedSize(g->W) && fastBlockedSize(g->H)) && !blockedGenericFits(g))
+    g->blocked = false;
   *out = g;
   return HWS_OK;
 }

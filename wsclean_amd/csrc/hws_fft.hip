@@ -992,50 +992,58 @@ template <int SIGN, int MODE, bool REF, int NT>
 __global__ void __launch_bounds__(NT)
 fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
                             unsigned long long radices, const float2* __restrict__ tw,
-                            GenericSparseArgs sp, ModelSource ms) {
+                            GenericSparseArgs sp, ModelSource ms, int rbShift) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int pitch = W + 1;
+  const int rb = 1 << rbShift;            // rows of this workgroup: 4, or 2 (half a row block)
+  const int r0 = int(blockIdx.x) << rbShift;
+  const int f0 = r0 & 3;                  // their position in the four-row block
   const int layer = sp.layer0 + int(blockIdx.y);
-  const int ty = (int(blockIdx.x) * 4) >> sp.tshift;
+  const int ty = r0 >> sp.tshift;
   const size_t rowBase = (size_t(layer) * sp.tilesY + ty) * sp.tilesX;
   if (MODE == 1) {
     int any = 0;
     for (int t = tid; t < sp.tilesX; t += nthr) any |= bitAt(sp.bits, rowBase + t);
     if (!__syncthreads_or(any)) return;
   }
-  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(blockIdx.x) * 4 * W;
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(r0 >> 2) * 4 * W;
   const double w = MODE == 2 ? layerToW(ms.map, ms.layer0 + int(blockIdx.y)) : 0.0;
-  for (int i = tid; i < 4 * W; i += nthr) {
-    const int x = i >> 2, f = i & 3;
+  for (int i = tid; i < rb * W; i += nthr) {
+    const int x = i >> rbShift, fl = i & (rb - 1);
     float2 v;
     if (MODE == 2)
-      v = modelCell<REF>(ms, w, x, int(blockIdx.x) * 4 + f, W, H);
+      v = modelCell<REF>(ms, w, x, r0 + fl, W, H);
     else if (MODE == 1 && !bitAt(sp.bits, rowBase + (x >> sp.tshift)))
       v = make_float2(0.0f, 0.0f);
     else
-      v = base[i];
-    lds[f * pitch + x] = v;
+      v = base[size_t(x) * 4 + f0 + fl];
+    lds[fl * pitch + x] = v;
   }
   __syncthreads();
-  fftLds<SIGN>(lds, pitch, 4, W, radices, tw, tid, nthr);
-  for (int i = tid; i < 4 * W; i += nthr) base[i] = lds[(i & 3) * pitch + (i >> 2)];
+  fftLds<SIGN>(lds, pitch, rb, W, radices, tw, tid, nthr);
+  for (int i = tid; i < rb * W; i += nthr) {
+    const int x = i >> rbShift, fl = i & (rb - 1);
+    base[size_t(x) * 4 + f0 + fl] = lds[fl * pitch + x];
+  }
 }
 
 template <int SIGN, int MODE, int NT>
 __global__ void __launch_bounds__(NT)
 fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlocks,
                             unsigned long long radices, const float2* __restrict__ tw,
-                            GenericSparseArgs sp) {
+                            GenericSparseArgs sp, int cbShift) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const int cb = 1 << cbShift;  // columns of this workgroup: 4, or 2
   int blk = blockIdx.x;
   // (the sparse forward mode keeps the natural order: its live column blocks
   // lie near the uv origin, at both ends of the row)
   if (MODE != 2 && (nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
   const int pitch = H + 1;
   const int layer = sp.layer0 + int(blockIdx.y);
-  const int tx = (blk * 4) >> sp.tshift;
+  const int c0 = blk << cbShift;
+  const int tx = c0 >> sp.tshift;
   // MODE 1: bit (layer, tile row); MODE 2: bit (layer, tile row, this tile column)
   auto live = [&](int y) -> bool {
     const int ty = y >> sp.tshift;
@@ -1047,20 +1055,21 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
     for (int t = tid; t < sp.tilesY; t += nthr) any |= live(t << sp.tshift);
     if (!__syncthreads_or(any)) return;
   }
-  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(blk) * 16;
-  // element i: row block i >> 4, column (i >> 2) & 3, row in block i & 3
-  for (int i = tid; i < 4 * H; i += nthr) {
-    const int yb = i >> 4, c = (i >> 2) & 3, r = i & 3;
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(c0) * 4;
+  // element i: row block i >> (cbShift + 2), column (i >> 2) & (cb - 1), row
+  // in block i & 3
+  for (int i = tid; i < cb * H; i += nthr) {
+    const int yb = i >> (cbShift + 2), c = (i >> 2) & (cb - 1), r = i & 3;
     float2 v = make_float2(0.0f, 0.0f);
-    if (MODE != 1 || live(yb * 4 + r)) v = base[size_t(yb) * W * 4 + (i & 15)];
+    if (MODE != 1 || live(yb * 4 + r)) v = base[size_t(yb) * W * 4 + size_t(c) * 4 + r];
     lds[c * pitch + yb * 4 + r] = v;
   }
   __syncthreads();
-  fftLds<SIGN>(lds, pitch, 4, H, radices, tw, tid, nthr);
-  for (int i = tid; i < 4 * H; i += nthr) {
-    const int yb = i >> 4, c = (i >> 2) & 3, r = i & 3;
+  fftLds<SIGN>(lds, pitch, cb, H, radices, tw, tid, nthr);
+  for (int i = tid; i < cb * H; i += nthr) {
+    const int yb = i >> (cbShift + 2), c = (i >> 2) & (cb - 1), r = i & 3;
     if (MODE == 2 && !live(yb * 4 + r)) continue;
-    base[size_t(yb) * W * 4 + (i & 15)] = lds[c * pitch + yb * 4 + r];
+    base[size_t(yb) * W * 4 + size_t(c) * 4 + r] = lds[c * pitch + yb * 4 + r];
   }
 }
 
@@ -1071,31 +1080,31 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
 template <int SIGN, int MODE, bool REF>
 void launchRowsGeneric(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
                        int H, unsigned long long rad, const float2* tw, GenericSparseArgs sp,
-                       ModelSource ms) {
+                       ModelSource ms, int rbShift) {
   if (threads <= 256)
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 256>), grid,
-                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms);
+                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms, rbShift);
   else if (threads <= 512)
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 512>), grid,
-                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms);
+                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms, rbShift);
   else
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 1024>), grid,
-                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms);
+                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms, rbShift);
 }
 
 template <int SIGN, int MODE>
 void launchColsGeneric(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
                        int H, int nColBlocks, unsigned long long rad, const float2* tw,
-                       GenericSparseArgs sp) {
+                       GenericSparseArgs sp, int cbShift) {
   if (threads <= 256)
     hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 256>), grid, dim3(threads), lds,
-                       st, data, W, H, nColBlocks, rad, tw, sp);
+                       st, data, W, H, nColBlocks, rad, tw, sp, cbShift);
   else if (threads <= 512)
     hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 512>), grid, dim3(threads), lds,
-                       st, data, W, H, nColBlocks, rad, tw, sp);
+                       st, data, W, H, nColBlocks, rad, tw, sp, cbShift);
   else
     hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 1024>), grid, dim3(threads), lds,
-                       st, data, W, H, nColBlocks, rad, tw, sp);
+                       st, data, W, H, nColBlocks, rad, tw, sp, cbShift);
 }
 
 template <int SIGN>
@@ -1262,12 +1271,36 @@ bool launchBlocked(int n, bool rows, float2* data, int W, int H, int nbatch,
 // Sizes with compile-time blocked kernels (and their sparse variants).
 bool fastBlockedSize(int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0; }
 
-// The blocked layout is used whenever both sizes are multiples of 4 and four
-// rows or columns fit in 64 KiB of LDS (n <= 2044) for the runtime-radix
-// kernels.
+// LDS a runtime-radix workgroup may take (the device's 160 KiB per workgroup,
+// less room for the kernels' static arrays).
+constexpr size_t kGenericLdsBytes = size_t(152) * 1024;
+
+// The blocked layout is used whenever both sizes are multiples of 4 and two
+// lines of each fit the LDS of one runtime-radix workgroup (n <= 9724); the
+// transform plans then also have to fit the workgroup's threads
+// (blockedGenericFits).
 bool blockedLayoutSupported(int W, int H) {
   if (fastBlockedSize(W) && fastBlockedSize(H)) return true;
-  return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && W <= 2044 && H <= 2044;
+  auto fits = [](int n) { return size_t(2) * (n + 1) * sizeof(float2) <= kGenericLdsBytes; };
+  return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && fits(W) && fits(H);
+}
+
+namespace {
+// log2 of the lines (rows or columns) a runtime-radix workgroup of the
+// blocked layout transforms: the four of a row block where [4][n + 1] values
+// fit the LDS and the plan the threads, else two; -1 if two do not fit.
+int genericLinesShift(const FftPlan& plan, bool forceTwo) {
+  auto fits = [&](int lines) {
+    return size_t(lines) * (plan.n + 1) * sizeof(float2) <= kGenericLdsBytes &&
+           threadsFor(plan, lines) != 0;
+  };
+  if (!forceTwo && fits(4)) return 2;
+  return fits(2) ? 1 : -1;
+}
+}  // namespace
+
+bool blockedGenericFits(const hws_gridder* g) {
+  return genericLinesShift(g->rowPlan, false) >= 0 && genericLinesShift(g->colPlan, false) >= 0;
 }
 
 namespace {
@@ -1340,13 +1373,16 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     // kernels)
     const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
     const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
-    const int tr = threadsFor(g->rowPlan, 4), tc = threadsFor(g->colPlan, 4);
-    if (tr == 0 || tc == 0)
+    // four rows (columns) per workgroup where they fit the LDS, else two
+    const int rbShift = genericLinesShift(g->rowPlan, g->forceTwoLines);
+    const int cbShift = genericLinesShift(g->colPlan, g->forceTwoLines);
+    if (rbShift < 0 || cbShift < 0)
       return setLastError(g, HWS_ERR_UNSUPPORTED, "transform does not fit one workgroup");
-    const size_t ldsR = size_t(4) * (W + 1) * sizeof(float2);
-    const size_t ldsC = size_t(4) * (H + 1) * sizeof(float2);
+    const int tr = threadsFor(g->rowPlan, 1 << rbShift), tc = threadsFor(g->colPlan, 1 << cbShift);
+    const size_t ldsR = size_t(1 << rbShift) * (W + 1) * sizeof(float2);
+    const size_t ldsC = size_t(1 << cbShift) * (H + 1) * sizeof(float2);
     const unsigned long long radR = packRadices(g->rowPlan), radC = packRadices(g->colPlan);
-    const int nColBlocks = W / 4;
+    const int nColBlocks = W >> cbShift;
     GenericSparseArgs rs{}, cs{};
     ModelSource msrc{};
     if (sparse) {
@@ -1362,27 +1398,27 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
       }
     }
     if (model) msrc = *model;
-    const dim3 gr(H / 4, nbatch), gc(nColBlocks, nbatch);
+    const dim3 gr(H >> rbShift, nbatch), gc(nColBlocks, nbatch);
     hipStream_t st = g->stream;
     if (sign > 0) {
       if (sparse) {
-        launchRowsGeneric<+1, 1, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
-        launchColsGeneric<+1, 1>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
+        launchRowsGeneric<+1, 1, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
+        launchColsGeneric<+1, 1>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
       } else {
-        launchRowsGeneric<+1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
-        launchColsGeneric<+1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
+        launchRowsGeneric<+1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
+        launchColsGeneric<+1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
       }
     } else {
       if (model && model->precision == HWS_PRECISION_REFERENCE)
-        launchRowsGeneric<-1, 2, true>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchRowsGeneric<-1, 2, true>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
       else if (model)
-        launchRowsGeneric<-1, 2, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchRowsGeneric<-1, 2, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
       else
-        launchRowsGeneric<-1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchRowsGeneric<-1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
       if (sparse)
-        launchColsGeneric<-1, 2>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
+        launchColsGeneric<-1, 2>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
       else
-        launchColsGeneric<-1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
+        launchColsGeneric<-1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
     }
     return checkHip(g, hipGetLastError(), "fft blocked generic launch");
   }

@@ -385,7 +385,8 @@ struct hws_gridder {
   // (HWS_NO_MFMA=1 in the environment; for comparisons)
   bool noMfma = false;
   bool noF16Mfma = false;
-  int degridBlocksPerCu = 5;  // the mixed precision grids with the FP32 matrix-core kernel
+  int degridBlocksPerCu = 5;
+  bool forceTwoLines = false;  // runtime-radix transforms: two lines per workgroup  // the mixed precision grids with the FP32 matrix-core kernel
   // run the backward columns pass and the projection apart, and the forward
   // rows pass four layers per workgroup (the forms the fused kernels replace)
   bool noFusedProject = false;
@@ -566,6 +567,9 @@ __host__ __device__ inline void cellOf(size_t p, int W, bool blocked, int& x, in
 }
 
 bool blockedLayoutSupported(int W, int H);
+// Whether the runtime-radix transforms of the blocked layout fit one
+// workgroup at this gridder's sizes (needs the row and column plans).
+bool blockedGenericFits(const hws_gridder* g);
 bool fastBlockedSize(int n);
 
 inline int passLayerStart(const hws_gridder* g, int pass) {
