@@ -117,9 +117,32 @@ struct Butterfly {
 
 constexpr int kMaxValues = 16;  // complex values held per thread per stage
 
+// LDS position of element e of a line of the runtime-radix transforms: one
+// pad value after every 16, so that the first stage's writes (stride R = 16,
+// 8, 4 or 2 values across the lanes) fall in distinct banks.
+__device__ __forceinline__ int lp(int e) { return e + (e >> 4); }
+
+// Line pitch (in values) of those transforms for length n: the padded length
+// plus one.
+__host__ __device__ constexpr int ldsPitch(int n) { return n + (n >> 4) + 1; }
+
+// Division by a divisor that is the same on every lane: x / d ==
+// umulhi(x, ceil(2^32 / d)) for d >= 2 whenever x * d < 2^32 (here x and d
+// stay below 2^16).
+struct UniformDiv {
+  uint32_t m;
+  int d;
+};
+__device__ __forceinline__ UniformDiv uniformDiv(int d) {
+  return UniformDiv{d > 1 ? 0xFFFFFFFFu / uint32_t(d) + 1u : 0u, d};
+}
+__device__ __forceinline__ int divBy(int x, UniformDiv u) {
+  return u.d == 1 ? x : int(__umulhi(uint32_t(x), u.m));
+}
+
 // One Stockham stage of radix R over nfft transforms of length n stored at
-// buf + f * pitch. Butterfly b of transform f reads x[b + r*B] (B = n/R),
-// applies twiddle w^(jm*r) with jm = b % ns, and writes
+// buf + f * pitch (element e at lp(e)). Butterfly b of transform f reads
+// x[b + r*B] (B = n/R), applies twiddle w^(jm*r) with jm = b % ns, and writes
 // y[(b/ns)*ns*R + jm + r*ns].
 template <int SIGN, int R>
 __device__ __forceinline__ void stockhamStage(float2* buf, int n, int pitch,
@@ -127,20 +150,26 @@ __device__ __forceinline__ void stockhamStage(float2* buf, int n, int pitch,
                                               const float2* __restrict__ tw,
                                               int tid, int nthr) {
   constexpr int kBatches = kMaxValues / R;
+  // The batch indices below depend only on (tid, n, R, nthr), so the compiler
+  // would hoist those of every radix out of the stage loop of fftLds and keep
+  // them all live (208 VGPRs, spills at 1024 threads); an opaque tid makes
+  // each stage compute its own.
+  asm volatile("" : "+v"(tid));
   const int B = n / R;
   const int total = B * nfft;
   const int twStep = n / (ns * R);
+  const UniformDiv byB = uniformDiv(B), byNs = uniformDiv(ns);
   float2 v[kBatches][R];
 #pragma unroll
   for (int q = 0; q < kBatches; ++q) {
     const int t = tid + q * nthr;
     if (t < total) {
-      const int f = t / B, b = t - f * B;
-      const int jm = b % ns;
+      const int f = divBy(t, byB), b = t - f * B;
+      const int jm = b - divBy(b, byNs) * ns;
       const float2* x = buf + f * pitch;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 val = x[b + r * B];
+        float2 val = x[lp(b + r * B)];
         if (r != 0 && jm != 0) val = cmul(val, tw[jm * r * twStep]);
         v[q][r] = val;
       }
@@ -151,13 +180,13 @@ __device__ __forceinline__ void stockhamStage(float2* buf, int n, int pitch,
   for (int q = 0; q < kBatches; ++q) {
     const int t = tid + q * nthr;
     if (t < total) {
-      const int f = t / B, b = t - f * B;
+      const int f = divBy(t, byB), b = t - f * B;
       Butterfly<SIGN, R>::run(v[q]);
-      const int jm = b % ns;
-      const int idxD = (b / ns) * ns * R + jm;
+      const int bq = divBy(b, byNs), jm = b - bq * ns;
+      const int idxD = bq * ns * R + jm;
       float2* y = buf + f * pitch;
 #pragma unroll
-      for (int r = 0; r < R; ++r) y[idxD + r * ns] = v[q][r];
+      for (int r = 0; r < R; ++r) y[lp(idxD + r * ns)] = v[q][r];
     }
   }
   __syncthreads();
@@ -165,6 +194,8 @@ __device__ __forceinline__ void stockhamStage(float2* buf, int n, int pitch,
 
 // Stage radices are packed 4 bits per stage (stage 0 lowest) so that the
 // per-stage dispatch reads a scalar rather than an indexed argument array.
+// Lines are stored padded (element e of line f at buf[f * pitch + lp(e)],
+// pitch >= ldsPitch(n)).
 template <int SIGN>
 __device__ __forceinline__ void fftLds(float2* buf, int pitch, int nfft, int n,
                        unsigned long long radices, const float2* tw, int tid,
@@ -931,15 +962,17 @@ fftRowsKernel(float2* __restrict__ data, int W, int H, int rowsPerBlock,
   const int nrows = min(rowsPerBlock, H - row0);
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) +
                  size_t(row0) * size_t(W);
-  const int total = nrows * W;
-  for (int i = tid; i < total; i += nthr) lds[i] = base[i];
+  const int pitch = ldsPitch(W);
+  for (int r = 0; r < nrows; ++r)
+    for (int x = tid; x < W; x += nthr) lds[r * pitch + lp(x)] = base[size_t(r) * W + x];
   __syncthreads();
-  fftLds<SIGN>(lds, W, nrows, W, radices, tw, tid, nthr);
-  for (int i = tid; i < total; i += nthr) base[i] = lds[i];
+  fftLds<SIGN>(lds, pitch, nrows, W, radices, tw, tid, nthr);
+  for (int r = 0; r < nrows; ++r)
+    for (int x = tid; x < W; x += nthr) base[size_t(r) * W + x] = lds[r * pitch + lp(x)];
 }
 
 // columns: grid (nColBlocks, nbatch); block handles columns
-// [c0, c0 + bc) of one layer; LDS holds [bc][H + 1].
+// [c0, c0 + bc) of one layer; LDS holds [bc][ldsPitch(H)].
 template <int SIGN, int NT>
 __global__ void __launch_bounds__(NT)
 fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
@@ -952,25 +985,27 @@ fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
   if ((nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
   const int c0 = blk * bc;
   const int ncols = min(bc, W - c0);
-  const int pitch = H + 1;
+  const int pitch = ldsPitch(H);
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + c0;
   const int total = ncols * H;
   for (int i = tid; i < total; i += nthr) {
     const int y = i / ncols, c = i - y * ncols;
-    lds[c * pitch + y] = base[size_t(y) * W + c];
+    lds[c * pitch + lp(y)] = base[size_t(y) * W + c];
   }
   __syncthreads();
   fftLds<SIGN>(lds, pitch, ncols, H, radices, tw, tid, nthr);
   for (int i = tid; i < total; i += nthr) {
     const int y = i / ncols, c = i - y * ncols;
-    base[size_t(y) * W + c] = lds[c * pitch + y];
+    base[size_t(y) * W + c] = lds[c * pitch + lp(y)];
   }
 }
 
 // Blocked layout at sizes without a compile-time kernel (any 7-smooth size
 // with W and H multiples of 4): one workgroup per 4 rows (row pass) or 4
 // columns (column pass), read and written as whole 128-byte segments of the
-// blocked layout, transformed in LDS ([4][n + 1]) by the mixed-radix stages.
+// blocked layout (two rows or columns, 64-byte segments, where four do not
+// fit the LDS), transformed in LDS ([lines][ldsPitch(n)]) by the mixed-radix
+// stages.
 // The sparse modes mirror the compile-time kernels:
 //   rows MODE 1 (backward): only tiles with samples are read (bits: the
 //     occupancy of (layer, tile), sp.occ); a row block without any is
@@ -995,7 +1030,7 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
                             GenericSparseArgs sp, ModelSource ms, int rbShift) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int pitch = W + 1;
+  const int pitch = ldsPitch(W);
   const int rb = 1 << rbShift;            // rows of this workgroup: 4, or 2 (half a row block)
   const int r0 = int(blockIdx.x) << rbShift;
   const int f0 = r0 & 3;                  // their position in the four-row block
@@ -1018,13 +1053,13 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
       v = make_float2(0.0f, 0.0f);
     else
       v = base[size_t(x) * 4 + f0 + fl];
-    lds[fl * pitch + x] = v;
+    lds[fl * pitch + lp(x)] = v;
   }
   __syncthreads();
   fftLds<SIGN>(lds, pitch, rb, W, radices, tw, tid, nthr);
   for (int i = tid; i < rb * W; i += nthr) {
     const int x = i >> rbShift, fl = i & (rb - 1);
-    base[size_t(x) * 4 + f0 + fl] = lds[fl * pitch + x];
+    base[size_t(x) * 4 + f0 + fl] = lds[fl * pitch + lp(x)];
   }
 }
 
@@ -1040,7 +1075,7 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
   // (the sparse forward mode keeps the natural order: its live column blocks
   // lie near the uv origin, at both ends of the row)
   if (MODE != 2 && (nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
-  const int pitch = H + 1;
+  const int pitch = ldsPitch(H);
   const int layer = sp.layer0 + int(blockIdx.y);
   const int c0 = blk << cbShift;
   const int tx = c0 >> sp.tshift;
@@ -1062,14 +1097,14 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
     const int yb = i >> (cbShift + 2), c = (i >> 2) & (cb - 1), r = i & 3;
     float2 v = make_float2(0.0f, 0.0f);
     if (MODE != 1 || live(yb * 4 + r)) v = base[size_t(yb) * W * 4 + size_t(c) * 4 + r];
-    lds[c * pitch + yb * 4 + r] = v;
+    lds[c * pitch + lp(yb * 4 + r)] = v;
   }
   __syncthreads();
   fftLds<SIGN>(lds, pitch, cb, H, radices, tw, tid, nthr);
   for (int i = tid; i < cb * H; i += nthr) {
     const int yb = i >> (cbShift + 2), c = (i >> 2) & (cb - 1), r = i & 3;
     if (MODE == 2 && !live(yb * 4 + r)) continue;
-    base[size_t(yb) * W * 4 + size_t(c) * 4 + r] = lds[c * pitch + yb * 4 + r];
+    base[size_t(yb) * W * 4 + size_t(c) * 4 + r] = lds[c * pitch + lp(yb * 4 + r)];
   }
 }
 
@@ -1276,22 +1311,22 @@ bool fastBlockedSize(int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0
 constexpr size_t kGenericLdsBytes = size_t(152) * 1024;
 
 // The blocked layout is used whenever both sizes are multiples of 4 and two
-// lines of each fit the LDS of one runtime-radix workgroup (n <= 9724); the
+// lines of each fit the LDS of one runtime-radix workgroup (n <= 9152); the
 // transform plans then also have to fit the workgroup's threads
 // (blockedGenericFits).
 bool blockedLayoutSupported(int W, int H) {
   if (fastBlockedSize(W) && fastBlockedSize(H)) return true;
-  auto fits = [](int n) { return size_t(2) * (n + 1) * sizeof(float2) <= kGenericLdsBytes; };
+  auto fits = [](int n) { return size_t(2) * ldsPitch(n) * sizeof(float2) <= kGenericLdsBytes; };
   return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && fits(W) && fits(H);
 }
 
 namespace {
 // log2 of the lines (rows or columns) a runtime-radix workgroup of the
-// blocked layout transforms: the four of a row block where [4][n + 1] values
+// blocked layout transforms: the four of a row block where [4][ldsPitch(n)] values
 // fit the LDS and the plan the threads, else two; -1 if two do not fit.
 int genericLinesShift(const FftPlan& plan, bool forceTwo) {
   auto fits = [&](int lines) {
-    return size_t(lines) * (plan.n + 1) * sizeof(float2) <= kGenericLdsBytes &&
+    return size_t(lines) * ldsPitch(plan.n) * sizeof(float2) <= kGenericLdsBytes &&
            threadsFor(plan, lines) != 0;
   };
   if (!forceTwo && fits(4)) return 2;
@@ -1379,8 +1414,8 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     if (rbShift < 0 || cbShift < 0)
       return setLastError(g, HWS_ERR_UNSUPPORTED, "transform does not fit one workgroup");
     const int tr = threadsFor(g->rowPlan, 1 << rbShift), tc = threadsFor(g->colPlan, 1 << cbShift);
-    const size_t ldsR = size_t(1 << rbShift) * (W + 1) * sizeof(float2);
-    const size_t ldsC = size_t(1 << cbShift) * (H + 1) * sizeof(float2);
+    const size_t ldsR = size_t(1 << rbShift) * ldsPitch(W) * sizeof(float2);
+    const size_t ldsC = size_t(1 << cbShift) * ldsPitch(H) * sizeof(float2);
     const unsigned long long radR = packRadices(g->rowPlan), radC = packRadices(g->colPlan);
     const int nColBlocks = W >> cbShift;
     GenericSparseArgs rs{}, cs{};
@@ -1573,7 +1608,7 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     int rows = std::max(1, 4096 / W);
     rows = std::min(rows, H);
     const int threads = threadsFor(g->rowPlan, rows);
-    const size_t lds = size_t(rows) * W * sizeof(float2);
+    const size_t lds = size_t(rows) * ldsPitch(W) * sizeof(float2);
     if (threads == 0 || lds > size_t(maxLds))
       return setLastError(g, HWS_ERR_UNSUPPORTED, "row transform does not fit one workgroup");
     dim3 grid((H + rows - 1) / rows, nbatch);
@@ -1586,14 +1621,14 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     hws_status st = checkHip(g, hipGetLastError(), "fft rows launch");
     if (st != HWS_OK) return st;
   }
-  // columns: blocks of bc columns, [bc][H + 1] in LDS
+  // columns: blocks of bc columns, [bc][ldsPitch(H)] in LDS
   if (!colsDone) {
     int bc = std::max(1, 8192 / H);
     bc = std::min(std::min(bc, 16), W);
-    while (bc > 1 && size_t(bc) * (H + 1) * sizeof(float2) > size_t(maxLds)) --bc;
+    while (bc > 1 && size_t(bc) * ldsPitch(H) * sizeof(float2) > size_t(maxLds)) --bc;
     int threads = threadsFor(g->colPlan, bc);
     while (threads == 0 && bc > 1) threads = threadsFor(g->colPlan, --bc);
-    const size_t lds = size_t(bc) * size_t(H + 1) * sizeof(float2);
+    const size_t lds = size_t(bc) * size_t(ldsPitch(H)) * sizeof(float2);
     if (threads == 0 || lds > size_t(maxLds))
       return setLastError(g, HWS_ERR_UNSUPPORTED, "column transform does not fit one workgroup");
     const int nColBlocks = (W + bc - 1) / bc;
