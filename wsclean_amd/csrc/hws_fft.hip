@@ -1004,8 +1004,8 @@ fftColsKernel(float2* __restrict__ data, int W, int H, int bc, int nColBlocks,
 // with W and H multiples of 4): one workgroup per 4 rows (row pass) or 4
 // columns (column pass), read and written as whole 128-byte segments of the
 // blocked layout (two rows or columns, 64-byte segments, where four do not
-// fit the LDS), transformed in LDS ([lines][ldsPitch(n)]) by the mixed-radix
-// stages.
+// fit the LDS), by the mixed-radix stages: the first reads memory, the middle
+// ones exchange through the lines interleaved in LDS, the last writes memory.
 // The sparse modes mirror the compile-time kernels:
 //   rows MODE 1 (backward): only tiles with samples are read (bits: the
 //     occupancy of (layer, tile), sp.occ); a row block without any is
@@ -1023,16 +1023,152 @@ __device__ __forceinline__ bool bitAt(const uint32_t* bits, size_t b) {
   return (bits[b >> 5] >> (b & 31)) & 1u;
 }
 
-template <int SIGN, int MODE, bool REF, int NT>
-__global__ void __launch_bounds__(NT)
+// LDS position of value i of the interleaved lines of a blocked runtime-radix
+// workgroup (element e of line f is value e * L + f): one pad value after
+// every 16, so that the first stage's writes (stride R * L values across the
+// lanes) fall in distinct banks.
+__device__ __forceinline__ int ipad(int i) { return i + (i >> 4); }
+
+// LDS bytes of L interleaved lines of length n.
+__host__ __device__ constexpr size_t blockedGenericLds(int L, int n) {
+  return (size_t(L) * n + (size_t(L) * n >> 4) + 1) * sizeof(float2);
+}
+
+// One Stockham stage of radix R over the L interleaved lines of length n
+// (butterfly t: line f = t % L, index b = t / L, so the lanes of a wave read
+// and write the L lines of consecutive positions together). Butterfly b reads
+// x[b + r*B] (B = n/R), applies twiddle w^(jm*r) with jm = b % ns, and writes
+// y[(b/ns)*ns*R + jm + r*ns]. The first stage (ns == 1) reads its inputs with
+// load(pos, f) instead of from LDS and needs no barrier before its writes;
+// the last one writes its outputs with store(pos, f, v) instead of to LDS.
+template <int SIGN, int R, int L, bool FIRST, bool LAST, class Load, class Store>
+__device__ __forceinline__ void blockedRtStage(float2* lds, int n, int ns,
+                                               const float2* __restrict__ tw, int tid, int nthr,
+                                               const Load& load, const Store& store) {
+  constexpr int kBatches = kMaxValues / R;
+  constexpr int LS = L == 4 ? 2 : 1;
+  static_assert(L == 2 || L == 4, "lines");
+  // (an opaque tid keeps the compiler from hoisting the batch indices of
+  // every radix out of the stage loop, which keeps them all live)
+  asm volatile("" : "+v"(tid));
+  const int B = n / R;
+  const int total = B * L;
+  const int twStep = n / (ns * R);
+  const UniformDiv byNs = uniformDiv(ns);
+  float2 v[kBatches][R];
+#pragma unroll
+  for (int q = 0; q < kBatches; ++q) {
+    const int t = tid + q * nthr;
+    if (t < total) {
+      const int f = t & (L - 1), b = t >> LS;
+      const int jm = FIRST ? 0 : b - divBy(b, byNs) * ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int pos = b + r * B;
+        float2 x;
+        if (FIRST) {
+          x = load(pos, f);
+        } else {
+          x = lds[ipad(pos * L + f)];
+          if (r != 0 && jm != 0) x = cmul(x, tw[jm * r * twStep]);
+        }
+        v[q][r] = x;
+      }
+    }
+  }
+  if (!FIRST) __syncthreads();  // every read of this stage precedes its writes
+#pragma unroll
+  for (int q = 0; q < kBatches; ++q) {
+    const int t = tid + q * nthr;
+    if (t < total) {
+      const int f = t & (L - 1), b = t >> LS;
+      Butterfly<SIGN, R>::run(v[q]);
+      const int bq = FIRST ? b : divBy(b, byNs);
+      const int idxD = bq * ns * R + (b - bq * ns);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int pos = idxD + r * ns;
+        if (LAST)
+          store(pos, f, v[q][r]);
+        else
+          lds[ipad(pos * L + f)] = v[q][r];
+      }
+    }
+  }
+  if (!LAST) __syncthreads();
+}
+
+template <int SIGN, int L, bool FIRST, bool LAST, class Load, class Store>
+__device__ __forceinline__ void blockedRtStageR(int R, float2* lds, int n, int ns,
+                                                const float2* __restrict__ tw, int tid, int nthr,
+                                                const Load& load, const Store& store) {
+  switch (R) {
+#define HWS_RT_STAGE(RR)                                                                    \
+  case RR:                                                                                  \
+    blockedRtStage<SIGN, RR, L, FIRST, LAST>(lds, n, ns, tw, tid, nthr, load, store);       \
+    break;
+    HWS_RT_STAGE(16)
+    HWS_RT_STAGE(8)
+    HWS_RT_STAGE(4)
+    HWS_RT_STAGE(2)
+    HWS_RT_STAGE(3)
+    HWS_RT_STAGE(5)
+    HWS_RT_STAGE(7)
+#undef HWS_RT_STAGE
+    default: break;
+  }
+}
+
+// The whole transform of the L lines: the first stage from load, the middle
+// stages through LDS, the last stage to store (radices packed 4 bits per
+// stage, 16 packed as 1).
+template <int SIGN, int L, class Load, class Store>
+__device__ __forceinline__ void blockedRtFft(float2* lds, int n, unsigned long long radices,
+                                             const float2* __restrict__ tw, int tid, int nthr,
+                                             const Load& load, const Store& store) {
+  auto radixOf = [](unsigned long long rad) {
+    const int code = int(rad & 15ull);
+    return code == 1 ? 16 : code;
+  };
+  int R = radixOf(radices);
+  radices >>= 4;
+  auto toLds = [](int, int, float2) {};
+  auto fromLds = [](int, int) { return make_float2(0.0f, 0.0f); };
+  if (radices == 0ull) {  // a single stage: memory to memory
+    blockedRtStageR<SIGN, L, true, true>(R, lds, n, 1, tw, tid, nthr, load, store);
+    return;
+  }
+  blockedRtStageR<SIGN, L, true, false>(R, lds, n, 1, tw, tid, nthr, load, toLds);
+  int ns = R;
+  for (;;) {
+    R = radixOf(radices);
+    radices >>= 4;
+    if (radices == 0ull) break;
+    blockedRtStageR<SIGN, L, false, false>(R, lds, n, ns, tw, tid, nthr, fromLds, toLds);
+    ns *= R;
+  }
+  blockedRtStageR<SIGN, L, false, true>(R, lds, n, ns, tw, tid, nthr, fromLds, store);
+}
+
+// Two lines per workgroup: the two workgroups of a four-line block read and
+// write the two halves of the same 32-byte (rows) or 128-byte (columns)
+// segments. The hardware sends workgroup b to XCD b % 8, so within every 16
+// workgroups b and b + 8 are given the two halves of one block: they run on
+// the same XCD at about the same time and share its L2.
+__device__ __forceinline__ int pairedBlock(int b, int nBlocks) {
+  if ((nBlocks & 15) != 0) return b;
+  return (b & ~15) + ((b & 7) << 1) + ((b >> 3) & 1);
+}
+
+template <int SIGN, int MODE, bool REF, int L>
+__global__ void __launch_bounds__(1024)
 fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
                             unsigned long long radices, const float2* __restrict__ tw,
-                            GenericSparseArgs sp, ModelSource ms, int rbShift) {
+                            GenericSparseArgs sp, ModelSource ms) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int pitch = ldsPitch(W);
-  const int rb = 1 << rbShift;            // rows of this workgroup: 4, or 2 (half a row block)
-  const int r0 = int(blockIdx.x) << rbShift;
+  const int rblk = L == 2 ? pairedBlock(int(blockIdx.x), int(gridDim.x)) : int(blockIdx.x);
+  const int r0 = rblk * L;                // rows of this workgroup: 4, or 2 (half a row block)
   const int f0 = r0 & 3;                  // their position in the four-row block
   const int layer = sp.layer0 + int(blockIdx.y);
   const int ty = r0 >> sp.tshift;
@@ -1042,42 +1178,33 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
     for (int t = tid; t < sp.tilesX; t += nthr) any |= bitAt(sp.bits, rowBase + t);
     if (!__syncthreads_or(any)) return;
   }
-  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(r0 >> 2) * 4 * W;
+  float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(r0 >> 2) * 4 * W + f0;
   const double w = MODE == 2 ? layerToW(ms.map, ms.layer0 + int(blockIdx.y)) : 0.0;
-  for (int i = tid; i < rb * W; i += nthr) {
-    const int x = i >> rbShift, fl = i & (rb - 1);
-    float2 v;
-    if (MODE == 2)
-      v = modelCell<REF>(ms, w, x, r0 + fl, W, H);
-    else if (MODE == 1 && !bitAt(sp.bits, rowBase + (x >> sp.tshift)))
-      v = make_float2(0.0f, 0.0f);
-    else
-      v = base[size_t(x) * 4 + f0 + fl];
-    lds[fl * pitch + lp(x)] = v;
-  }
-  __syncthreads();
-  fftLds<SIGN>(lds, pitch, rb, W, radices, tw, tid, nthr);
-  for (int i = tid; i < rb * W; i += nthr) {
-    const int x = i >> rbShift, fl = i & (rb - 1);
-    base[size_t(x) * 4 + f0 + fl] = lds[fl * pitch + lp(x)];
-  }
+  auto load = [&](int x, int f) -> float2 {
+    if (MODE == 2) return modelCell<REF>(ms, w, x, r0 + f, W, H);
+    if (MODE == 1 && !bitAt(sp.bits, rowBase + (x >> sp.tshift))) return make_float2(0.0f, 0.0f);
+    return base[size_t(x) * 4 + f];
+  };
+  auto store = [&](int x, int f, float2 v) { base[size_t(x) * 4 + f] = v; };
+  blockedRtFft<SIGN, L>(lds, W, radices, tw, tid, nthr, load, store);
 }
 
-template <int SIGN, int MODE, int NT>
-__global__ void __launch_bounds__(NT)
+template <int SIGN, int MODE, int L>
+__global__ void __launch_bounds__(1024)
 fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlocks,
                             unsigned long long radices, const float2* __restrict__ tw,
-                            GenericSparseArgs sp, int cbShift) {
+                            GenericSparseArgs sp) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int cb = 1 << cbShift;  // columns of this workgroup: 4, or 2
   int blk = blockIdx.x;
   // (the sparse forward mode keeps the natural order: its live column blocks
   // lie near the uv origin, at both ends of the row)
-  if (MODE != 2 && (nColBlocks & 7) == 0) blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
-  const int pitch = ldsPitch(H);
+  if (MODE != 2 && (nColBlocks & 7) == 0)
+    blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
+  else if (MODE == 2 && L == 2)
+    blk = pairedBlock(blk, nColBlocks);
   const int layer = sp.layer0 + int(blockIdx.y);
-  const int c0 = blk << cbShift;
+  const int c0 = blk * L;  // columns of this workgroup: 4, or 2
   const int tx = c0 >> sp.tshift;
   // MODE 1: bit (layer, tile row); MODE 2: bit (layer, tile row, this tile column)
   auto live = [&](int y) -> bool {
@@ -1091,57 +1218,45 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
     if (!__syncthreads_or(any)) return;
   }
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(c0) * 4;
-  // element i: row block i >> (cbShift + 2), column (i >> 2) & (cb - 1), row
-  // in block i & 3
-  for (int i = tid; i < cb * H; i += nthr) {
-    const int yb = i >> (cbShift + 2), c = (i >> 2) & (cb - 1), r = i & 3;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (MODE != 1 || live(yb * 4 + r)) v = base[size_t(yb) * W * 4 + size_t(c) * 4 + r];
-    lds[c * pitch + lp(yb * 4 + r)] = v;
-  }
-  __syncthreads();
-  fftLds<SIGN>(lds, pitch, cb, H, radices, tw, tid, nthr);
-  for (int i = tid; i < cb * H; i += nthr) {
-    const int yb = i >> (cbShift + 2), c = (i >> 2) & (cb - 1), r = i & 3;
-    if (MODE == 2 && !live(yb * 4 + r)) continue;
-    base[size_t(yb) * W * 4 + size_t(c) * 4 + r] = lds[c * pitch + lp(yb * 4 + r)];
-  }
+  // element (y, column c0 + f): row block y >> 2, row y & 3 in it
+  auto addr = [&](int y, int f) { return size_t(y >> 2) * W * 4 + size_t(f) * 4 + (y & 3); };
+  auto load = [&](int y, int f) -> float2 {
+    if (MODE == 1 && !live(y)) return make_float2(0.0f, 0.0f);
+    return base[addr(y, f)];
+  };
+  auto store = [&](int y, int f, float2 v) {
+    if (MODE == 2 && !live(y)) return;
+    base[addr(y, f)] = v;
+  };
+  blockedRtFft<SIGN, L>(lds, H, radices, tw, tid, nthr, load, store);
 }
 
-// The runtime-radix kernels are compiled for three launch bounds; a launch
-// takes the smallest that holds its threads, so the compiler may give each
-// thread the registers its stages need (at 1024 threads it holds 128 and
-// spills).
 template <int SIGN, int MODE, bool REF>
-void launchRowsGeneric(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
-                       int H, unsigned long long rad, const float2* tw, GenericSparseArgs sp,
-                       ModelSource ms, int rbShift) {
-  if (threads <= 256)
-    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 256>), grid,
-                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms, rbShift);
-  else if (threads <= 512)
-    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 512>), grid,
-                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms, rbShift);
+void launchRowsGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_t st,
+                       float2* data, int W, int H, unsigned long long rad, const float2* tw,
+                       GenericSparseArgs sp, ModelSource ms) {
+  if (lines == 4)
+    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 4>), grid, dim3(threads),
+                       lds, st, data, W, H, rad, tw, sp, ms);
   else
-    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 1024>), grid,
-                       dim3(threads), lds, st, data, W, H, rad, tw, sp, ms, rbShift);
+    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 2>), grid, dim3(threads),
+                       lds, st, data, W, H, rad, tw, sp, ms);
 }
 
 template <int SIGN, int MODE>
-void launchColsGeneric(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
-                       int H, int nColBlocks, unsigned long long rad, const float2* tw,
-                       GenericSparseArgs sp, int cbShift) {
-  if (threads <= 256)
-    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 256>), grid, dim3(threads), lds,
-                       st, data, W, H, nColBlocks, rad, tw, sp, cbShift);
-  else if (threads <= 512)
-    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 512>), grid, dim3(threads), lds,
-                       st, data, W, H, nColBlocks, rad, tw, sp, cbShift);
+void launchColsGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_t st,
+                       float2* data, int W, int H, int nColBlocks, unsigned long long rad,
+                       const float2* tw, GenericSparseArgs sp) {
+  if (lines == 4)
+    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 4>), grid, dim3(threads), lds,
+                       st, data, W, H, nColBlocks, rad, tw, sp);
   else
-    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 1024>), grid, dim3(threads), lds,
-                       st, data, W, H, nColBlocks, rad, tw, sp, cbShift);
+    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 2>), grid, dim3(threads), lds,
+                       st, data, W, H, nColBlocks, rad, tw, sp);
 }
 
+// The plain (row-major) runtime-radix kernels are compiled for three launch
+// bounds; a launch takes the smallest that holds its threads.
 template <int SIGN>
 void launchRowsPlain(int threads, dim3 grid, size_t lds, hipStream_t st, float2* data, int W,
                      int H, int rows, unsigned long long rad, const float2* tw) {
@@ -1316,17 +1431,17 @@ constexpr size_t kGenericLdsBytes = size_t(152) * 1024;
 // (blockedGenericFits).
 bool blockedLayoutSupported(int W, int H) {
   if (fastBlockedSize(W) && fastBlockedSize(H)) return true;
-  auto fits = [](int n) { return size_t(2) * ldsPitch(n) * sizeof(float2) <= kGenericLdsBytes; };
+  auto fits = [](int n) { return blockedGenericLds(2, n) <= kGenericLdsBytes; };
   return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && fits(W) && fits(H);
 }
 
 namespace {
 // log2 of the lines (rows or columns) a runtime-radix workgroup of the
-// blocked layout transforms: the four of a row block where [4][ldsPitch(n)] values
-// fit the LDS and the plan the threads, else two; -1 if two do not fit.
+// blocked layout transforms: the four of a row block where they fit the LDS
+// and the plan the threads, else two; -1 if two do not fit.
 int genericLinesShift(const FftPlan& plan, bool forceTwo) {
   auto fits = [&](int lines) {
-    return size_t(lines) * ldsPitch(plan.n) * sizeof(float2) <= kGenericLdsBytes &&
+    return blockedGenericLds(lines, plan.n) <= kGenericLdsBytes &&
            threadsFor(plan, lines) != 0;
   };
   if (!forceTwo && fits(4)) return 2;
@@ -1414,8 +1529,8 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     if (rbShift < 0 || cbShift < 0)
       return setLastError(g, HWS_ERR_UNSUPPORTED, "transform does not fit one workgroup");
     const int tr = threadsFor(g->rowPlan, 1 << rbShift), tc = threadsFor(g->colPlan, 1 << cbShift);
-    const size_t ldsR = size_t(1 << rbShift) * ldsPitch(W) * sizeof(float2);
-    const size_t ldsC = size_t(1 << cbShift) * ldsPitch(H) * sizeof(float2);
+    const int rl = 1 << rbShift, cl = 1 << cbShift;
+    const size_t ldsR = blockedGenericLds(rl, W), ldsC = blockedGenericLds(cl, H);
     const unsigned long long radR = packRadices(g->rowPlan), radC = packRadices(g->colPlan);
     const int nColBlocks = W >> cbShift;
     GenericSparseArgs rs{}, cs{};
@@ -1437,23 +1552,23 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     hipStream_t st = g->stream;
     if (sign > 0) {
       if (sparse) {
-        launchRowsGeneric<+1, 1, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
-        launchColsGeneric<+1, 1>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
+        launchRowsGeneric<+1, 1, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchColsGeneric<+1, 1>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       } else {
-        launchRowsGeneric<+1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
-        launchColsGeneric<+1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
+        launchRowsGeneric<+1, 0, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
+        launchColsGeneric<+1, 0>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       }
     } else {
       if (model && model->precision == HWS_PRECISION_REFERENCE)
-        launchRowsGeneric<-1, 2, true>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
+        launchRowsGeneric<-1, 2, true>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       else if (model)
-        launchRowsGeneric<-1, 2, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
+        launchRowsGeneric<-1, 2, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       else
-        launchRowsGeneric<-1, 0, false>(tr, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc, rbShift);
+        launchRowsGeneric<-1, 0, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       if (sparse)
-        launchColsGeneric<-1, 2>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
+        launchColsGeneric<-1, 2>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       else
-        launchColsGeneric<-1, 0>(tc, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs, cbShift);
+        launchColsGeneric<-1, 0>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
     }
     return checkHip(g, hipGetLastError(), "fft blocked generic launch");
   }
