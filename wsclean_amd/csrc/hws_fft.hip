@@ -1039,8 +1039,10 @@ __host__ __device__ constexpr size_t blockedGenericLds(int L, int n) {
 // and write the L lines of consecutive positions together). Butterfly b reads
 // x[b + r*B] (B = n/R), applies twiddle w^(jm*r) with jm = b % ns, and writes
 // y[(b/ns)*ns*R + jm + r*ns]. The first stage (ns == 1) reads its inputs with
-// load(pos, f) instead of from LDS and needs no barrier before its writes;
-// the last one writes its outputs with store(pos, f, v) instead of to LDS.
+// load(slot, pos, f) instead of from LDS (slot = q * R + r, this thread's
+// q-th butterfly's r-th input, below kMaxValues) and needs no barrier before
+// its writes; the last one writes its outputs with store(pos, f, v) instead
+// of to LDS.
 template <int SIGN, int R, int L, bool FIRST, bool LAST, class Load, class Store>
 __device__ __forceinline__ void blockedRtStage(float2* lds, int n, int ns,
                                                const float2* __restrict__ tw, int tid, int nthr,
@@ -1067,7 +1069,7 @@ __device__ __forceinline__ void blockedRtStage(float2* lds, int n, int ns,
         const int pos = b + r * B;
         float2 x;
         if (FIRST) {
-          x = load(pos, f);
+          x = load(q * R + r, pos, f);
         } else {
           x = lds[ipad(pos * L + f)];
           if (r != 0 && jm != 0) x = cmul(x, tw[jm * r * twStep]);
@@ -1133,7 +1135,7 @@ __device__ __forceinline__ void blockedRtFft(float2* lds, int n, unsigned long l
   int R = radixOf(radices);
   radices >>= 4;
   auto toLds = [](int, int, float2) {};
-  auto fromLds = [](int, int) { return make_float2(0.0f, 0.0f); };
+  auto fromLds = [](int, int, int) { return make_float2(0.0f, 0.0f); };
   if (radices == 0ull) {  // a single stage: memory to memory
     blockedRtStageR<SIGN, L, true, true>(R, lds, n, 1, tw, tid, nthr, load, store);
     return;
@@ -1180,7 +1182,7 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
   }
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(r0 >> 2) * 4 * W + f0;
   const double w = MODE == 2 ? layerToW(ms.map, ms.layer0 + int(blockIdx.y)) : 0.0;
-  auto load = [&](int x, int f) -> float2 {
+  auto load = [&](int, int x, int f) -> float2 {
     if (MODE == 2) return modelCell<REF>(ms, w, x, r0 + f, W, H);
     if (MODE == 1 && !bitAt(sp.bits, rowBase + (x >> sp.tshift))) return make_float2(0.0f, 0.0f);
     return base[size_t(x) * 4 + f];
@@ -1220,7 +1222,7 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
   float2* base = data + size_t(blockIdx.y) * size_t(W) * size_t(H) + size_t(c0) * 4;
   // element (y, column c0 + f): row block y >> 2, row y & 3 in it
   auto addr = [&](int y, int f) { return size_t(y >> 2) * W * 4 + size_t(f) * 4 + (y & 3); };
-  auto load = [&](int y, int f) -> float2 {
+  auto load = [&](int, int y, int f) -> float2 {
     if (MODE == 1 && !live(y)) return make_float2(0.0f, 0.0f);
     return base[addr(y, f)];
   };
@@ -1229,6 +1231,93 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
     base[addr(y, f)] = v;
   };
   blockedRtFft<SIGN, L>(lds, H, radices, tw, tid, nthr, load, store);
+}
+
+// The first-stage cells of this thread in a blocked runtime-radix transform
+// whose first radix is R: visit(slot, pos, f) for each (the slots of
+// blockedRtStage's first stage).
+template <int R, int L, class Visit>
+__device__ __forceinline__ void firstStageCells(int n, int tid, int nthr, const Visit& visit) {
+  constexpr int kBatches = kMaxValues / R;
+  constexpr int LS = L == 4 ? 2 : 1;
+  asm volatile("" : "+v"(tid));
+  const int B = n / R;
+  const int total = B * L;
+#pragma unroll
+  for (int q = 0; q < kBatches; ++q) {
+    const int t = tid + q * nthr;
+    if (t < total) {
+      const int f = t & (L - 1), b = t >> LS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) visit(q * R + r, b + r * B, f);
+    }
+  }
+}
+
+// Forward rows pass computed from the model for every layer of the batch at
+// the sizes the runtime-radix kernels serve (fftRowsModelLayersKernel at the
+// power-of-two sizes): a workgroup owns L rows of one image, reads their
+// model values and computes their n' once into registers (the cells of its
+// first stage), and then, layer after layer, forms each cell's w-phased
+// input (modelTerm, as modelCell forms it), transforms the rows and writes
+// the tile columns the columns pass reads (colNeed: one bit per (layer, tile
+// column)). grid (H / L).
+template <bool REF, bool CPLX, int L>
+__global__ void __launch_bounds__(1024)
+fftRowsModelLayersGenericKernel(float2* __restrict__ data, int W, int H, int nbatch,
+                                unsigned long long radices, const float2* __restrict__ tw,
+                                ModelSource ms, const uint32_t* __restrict__ colNeed,
+                                int tshift) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int rblk = L == 2 ? pairedBlock(int(blockIdx.x), int(gridDim.x)) : int(blockIdx.x);
+  const int r0 = rblk * L;
+  const size_t HW = size_t(W) * H;
+  float2* rowBase = data + size_t(r0 >> 2) * 4 * W + (r0 & 3);
+  const int tilesX = W >> tshift;
+  float re[kMaxValues], imv[CPLX ? kMaxValues : 1];
+  double np[kMaxValues];
+  auto cell = [&](int slot, int x, int f) {
+    const int y = r0 + f;
+    int yDest = y + H / 2;
+    if (yDest >= H) yDest -= H;
+    int xDest = (W - x) + W / 2;
+    if (xDest >= W) xDest -= W;
+    const double m = pixelM(yDest, H, ms.dm, ms.mShift);
+    const double l = pixelL(xDest, W, ms.dl, ms.lShift);
+    double v = nPrimeSeries(l, m);
+    if (REF) v = double(float(v));
+    np[slot] = v;
+    const size_t src = size_t(yDest) * W + xDest;
+    re[slot] = ms.re[src];
+    if (CPLX) imv[slot] = -ms.im[src];
+  };
+  switch (radices & 15ull) {
+    case 1: firstStageCells<16, L>(W, tid, nthr, cell); break;
+    case 8: firstStageCells<8, L>(W, tid, nthr, cell); break;
+    case 4: firstStageCells<4, L>(W, tid, nthr, cell); break;
+    case 2: firstStageCells<2, L>(W, tid, nthr, cell); break;
+    case 3: firstStageCells<3, L>(W, tid, nthr, cell); break;
+    case 5: firstStageCells<5, L>(W, tid, nthr, cell); break;
+    case 7: firstStageCells<7, L>(W, tid, nthr, cell); break;
+    default: break;
+  }
+#pragma unroll 1
+  for (int b = 0; b < nbatch; ++b) {
+    const int layer = ms.layer0 + b;
+    const double w = layerToW(ms.map, layer);
+    float2* base = rowBase + size_t(b) * HW;
+    const size_t cbit = size_t(layer) * tilesX;
+    auto load = [&](int slot, int, int) -> float2 {
+      return modelTerm<REF>(re[slot], CPLX ? imv[slot] : 0.0f, CPLX, w, np[slot]);
+    };
+    auto store = [&](int x, int f, float2 v) {
+      const size_t bit = cbit + size_t(x >> tshift);
+      if ((colNeed[bit >> 5] >> (bit & 31)) & 1u) streamStore(&base[size_t(x) * 4 + f], v);
+    };
+    blockedRtFft<-1, L>(lds, W, radices, tw, tid, nthr, load, store);
+    __syncthreads();  // the last stage's LDS reads precede the next layer's writes
+  }
 }
 
 template <int SIGN, int MODE, bool REF>
@@ -1241,6 +1330,24 @@ void launchRowsGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_
   else
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 2>), grid, dim3(threads),
                        lds, st, data, W, H, rad, tw, sp, ms);
+}
+
+void launchModelLayersGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_t st,
+                              float2* data, int W, int H, int nbatch, unsigned long long rad,
+                              const float2* tw, const ModelSource& ms, const uint32_t* colNeed,
+                              int tshift) {
+  const bool cplx = ms.cplx != 0;
+#define HWS_MODEL_GENERIC(C, LL)                                                                \
+  hipLaunchKernelGGL((fftRowsModelLayersGenericKernel<false, C, LL>), grid, dim3(threads), lds,  \
+                     st, data, W, H, nbatch, rad, tw, ms, colNeed, tshift)
+  if (lines == 4) {
+    if (cplx) HWS_MODEL_GENERIC(true, 4);
+    else HWS_MODEL_GENERIC(false, 4);
+  } else {
+    if (cplx) HWS_MODEL_GENERIC(true, 2);
+    else HWS_MODEL_GENERIC(false, 2);
+  }
+#undef HWS_MODEL_GENERIC
 }
 
 template <int SIGN, int MODE>
@@ -1559,7 +1666,12 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
         launchColsGeneric<+1, 0>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       }
     } else {
-      if (model && model->precision == HWS_PRECISION_REFERENCE)
+      // (the mixed precision: the reference precision's FP64 sincos leaves no
+      // registers for the cells it keeps)
+      if (model && sparse && sparse->colNeed && model->precision != HWS_PRECISION_REFERENCE)
+        launchModelLayersGeneric(tr, rl, dim3(H >> rbShift), ldsR, st, data, W, H, nbatch, radR,
+                                 twRow, msrc, sparse->colNeed, sparse->tshift);
+      else if (model && model->precision == HWS_PRECISION_REFERENCE)
         launchRowsGeneric<-1, 2, true>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
       else if (model)
         launchRowsGeneric<-1, 2, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
