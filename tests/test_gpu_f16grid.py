@@ -1,7 +1,8 @@
 """The mixed precision's FP16 matrix-core grid kernel (gridMfmaF16Kernel:
 operands split into FP16 high and low parts, FP32 sums) against the FP32
-matrix-core kernel it replaces (selected with HWS_NO_F16_MFMA=1) and the
-oracle: the gridded layers of both agree to the FP32 kernel's own accuracy
+matrix-core kernel it replaces (selected with HWS_NO_F16_MFMA=1; at OS = 1023,
+where the FP16 kernel reads its padded table from device memory, the VALU
+cell kernel) and the oracle: the gridded layers of both agree to the FP32 kernel's own accuracy
 (relative L2 2e-6, the layer tolerance of the parity tests), also for
 visibilities spanning twelve decades and bins split into many work items,
 and the image stays within the mixed precision's tolerance of the oracle."""
@@ -23,12 +24,12 @@ def rel(a, b):
     return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
 
 
-def _layers(w, dl, uvw, wl, vis, nw, maxw, f16, chunk, complex_images=(1,)):
+def _layers(w, dl, uvw, wl, vis, nw, maxw, f16, chunk, complex_images=(1,), os_=63):
     import wsclean_amd
     old = os.environ.get("HWS_NO_F16_MFMA")
     os.environ["HWS_NO_F16_MFMA"] = "0" if f16 else "1"
     try:
-        g = wsclean_amd.WStackingGridder(w, w, dl, dl, 7, 63, precision="mixed",
+        g = wsclean_amd.WStackingGridder(w, w, dl, dl, 7, os_, precision="mixed",
                                          n_images=vis.shape[2], complex_images=complex_images,
                                          work_chunk=chunk)
     finally:
@@ -47,8 +48,11 @@ def _layers(w, dl, uvw, wl, vis, nw, maxw, f16, chunk, complex_images=(1,)):
     return lay, re, im
 
 
-@pytest.mark.parametrize("spread,chunk", [(0.0, 0), (12.0, 0), (3.0, 37)])
-def test_f16_grid_kernel_matches_fp32_kernel_and_oracle(oracle, spread, chunk):
+# OS = 1023: the padded table exceeds the LDS, the FP16 kernel reads it from
+# device memory, and the kernel it replaces is the VALU cell kernel
+@pytest.mark.parametrize("spread,chunk,os_", [(0.0, 0, 63), (12.0, 0, 63), (3.0, 37, 63),
+                                              (3.0, 0, 1023), (0.0, 37, 1023)])
+def test_f16_grid_kernel_matches_fp32_kernel_and_oracle(oracle, spread, chunk, os_):
     rng = np.random.default_rng(int(spread * 10) + chunk)
     w, nrow, nchan, nw = 128, 1200, 4, 6
     dl = 0.004
@@ -61,8 +65,8 @@ def test_f16_grid_kernel_matches_fp32_kernel_and_oracle(oracle, spread, chunk):
     vis = ((rng.standard_normal((nrow, nchan, 2)) + 1j * rng.standard_normal((nrow, nchan, 2)))
            * mag).astype(np.complex64)
     maxw = float(np.abs(uvw[:, 2]).max() / wl.min())
-    lay16, re16, im16 = _layers(w, dl, uvw, wl, vis, nw, maxw, True, chunk)
-    lay32, re32, im32 = _layers(w, dl, uvw, wl, vis, nw, maxw, False, chunk)
+    lay16, re16, im16 = _layers(w, dl, uvw, wl, vis, nw, maxw, True, chunk, os_=os_)
+    lay32, re32, im32 = _layers(w, dl, uvw, wl, vis, nw, maxw, False, chunk, os_=os_)
     worst = 0.0
     for k in range(2):
         for i in range(nw):
@@ -71,7 +75,7 @@ def test_f16_grid_kernel_matches_fp32_kernel_and_oracle(oracle, spread, chunk):
                 continue
             worst = max(worst, rel(lay16[k][i], lay32[k][i]))
     uvw_l = (uvw[:, None, :] / wl[None, :, None]).reshape(-1, 3)
-    g = oracle.RefGridder(w, w, dl, dl, 2, 7, 63, "float")
+    g = oracle.RefGridder(w, w, dl, dl, 2, 7, os_, "float")
     g.set_is_complex(False)
     g.prepare_w_layers(nw, 1e12, 0.0, maxw)
     g.start_inversion_pass(0)
@@ -82,7 +86,8 @@ def test_f16_grid_kernel_matches_fp32_kernel_and_oracle(oracle, spread, chunk):
     img_o = g.image()
     worst_o = max(rel(lay16[0][i], lay_o[i]) for i in range(nw) if np.abs(lay_o[i]).sum() > 0)
     e_img = rel(re16[0], img_o)
-    print(f"spread {spread} decades, chunk {chunk}: FP16 vs FP32 kernel layers {worst:.2e}; "
+    print(f"OS {os_}, spread {spread} decades, chunk {chunk}: FP16 vs FP32 kernel layers "
+          f"{worst:.2e}; "
           f"FP16 layers vs oracle {worst_o:.2e}, image {e_img:.2e} (FP32 kernel image "
           f"{rel(re32[0], img_o):.2e})")
     assert worst < 2e-6

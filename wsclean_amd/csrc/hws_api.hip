@@ -443,6 +443,7 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
   if ((st = makeFftPlan(g, g->H, g->colPlan)) != HWS_OK) return bail(st);
   if (g->blocked && !(fastBlockedSize(g->W) && fastBlockedSize(g->H)) && !blockedGenericFits(g))
     g->blocked = false;
+  if ((st = buildF16Table(g)) != HWS_OK) return bail(st);
   *out = g;
   return HWS_OK;
 }
@@ -455,6 +456,7 @@ void hws_destroy(hws_gridder* g) {
   if (g->userStream) (void)hipStreamSynchronize(g->stream);
   freeDev(g->dKernel);
   freeDev(g->dKernelPair);
+  freeDev(g->dF16Table);
   freeDev(g->dCorrXf);
   freeDev(g->dCorrYf);
   freeDev(g->dLayers);

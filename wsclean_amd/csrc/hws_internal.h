@@ -356,6 +356,9 @@ struct hws_gridder {
   float* dKernel = nullptr;          // [OS][K], tables[p][i]
   __half2* dKernelPair = nullptr;  // [OS][K] (hi, lo) FP16 pairs
   int kernelExp = 0;               // 2^kernelExp > the table's largest |value|
+  // FP16 matrix-core grid kernel's padded table in device memory (FP32
+  // values, then the (h, l) pairs), where it does not fit that kernel's LDS
+  float* dF16Table = nullptr;
   float* dCorrXf = nullptr;  // float-rounded D tables (float reference)
   float* dCorrYf = nullptr;
 
@@ -567,6 +570,9 @@ __host__ __device__ inline void cellOf(size_t p, int W, bool blocked, int& x, in
 }
 
 bool blockedLayoutSupported(int W, int H);
+// Builds dF16Table where the mixed precision's FP16 grid kernel reads its
+// padded table from device memory (no-op otherwise).
+hws_status buildF16Table(hws_gridder* g);
 // Whether the runtime-radix transforms of the blocked layout fit one
 // workgroup at this gridder's sizes (needs the row and column plans).
 bool blockedGenericFits(const hws_gridder* g);

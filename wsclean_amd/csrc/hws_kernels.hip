@@ -1490,6 +1490,35 @@ constexpr int kAExp = 14, kBExp = 13;
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Entry i of the FP16 grid kernel's padded table (rows of paddedRow(K, 16)
+// values, the kernel row at offset T - 1, zeros around it): the FP32 value,
+// and the (h, l) FP16 split of it times bScale packed in one word.
+__device__ __forceinline__ void f16TableEntry(int i, const float* __restrict__ ktab, int K,
+                                              int rowLen, float bScale, float& v,
+                                              uint32_t& pair) {
+#pragma clang fp contract(off)
+  constexpr int T = 16;
+  const int row = i / rowLen, c = i - row * rowLen - (T - 1);
+  v = 0.0f;
+  if (c >= 0 && c < K) v = ktab[row * K + c];
+  const float vs = v * bScale;
+  const _Float16 h = (_Float16)vs;
+  const _Float16 l = (_Float16)(vs - (float)h);
+  pair = uint32_t(__builtin_bit_cast(uint16_t, h)) |
+         (uint32_t(__builtin_bit_cast(uint16_t, l)) << 16);
+}
+
+// The padded table in device memory, for oversampling factors whose table
+// does not fit the kernel's LDS (read through the caches instead).
+__global__ void __launch_bounds__(kThreads)
+f16TableKernel(const float* __restrict__ ktab, int K, int OS, int eB, float* __restrict__ kt,
+               uint32_t* __restrict__ kp) {
+  const int rowLen = paddedRow(K, 16);
+  const int i = int(blockIdx.x) * kThreads + int(threadIdx.x);
+  if (i >= OS * rowLen) return;
+  f16TableEntry(i, ktab, K, rowLen, ldexpf(1.0f, kBExp - eB), kt[i], kp[i]);
+}
+
 // The FP16 roundings of x0 - h0 and x1 - h1 packed into one word, h0 and h1
 // the two halves of h (x - h is exact in FP32 where h is x rounded toward
 // zero; v_fma_mix forms it from the FP16 operand and rounds once). Written
@@ -1517,12 +1546,16 @@ __device__ __forceinline__ uint32_t lowParts(float x0, float x1, uint32_t h) {
 // Operand layout: lane i + 16 g supplies A[i][8 g + t] and B[8 g + t][i] in
 // its eight halves t (the same k order on both sides, so records may take
 // any k); D as for gridMfmaKernel. Staging, work queue and output as there.
-template <int NIMG, bool ACCUM, bool ROT>
+// GT: the padded table is read from device memory (gkt, gkp: f16TableKernel's
+// output) rather than built in LDS, for oversampling factors whose table does
+// not fit the LDS (above 221 at K = 7); the values are the same.
+template <int NIMG, bool ACCUM, bool ROT, bool GT>
 __global__ void __launch_bounds__(kMfmaThreads)
 gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
                   const uint32_t* __restrict__ extraStart, const uint32_t* __restrict__ itemStart,
                   const uint32_t* __restrict__ itemBin, const uint2* __restrict__ recs,
                   const float2* __restrict__ vis, const float* __restrict__ ktab,
+                  const float* __restrict__ gkt, const uint32_t* __restrict__ gkp,
                   float2* __restrict__ layers, float2* __restrict__ partials,
                   uint32_t* __restrict__ queue, int eB) {
 #pragma clang fp contract(off)
@@ -1530,24 +1563,18 @@ gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
   extern __shared__ float smem[];
   const int K = a.ga.K, OS = a.ga.OS;
   const int rowLen = paddedRow(K, T);
-  const int tableWords = (OS * rowLen + 3) & ~3;
-  const float bScale = ldexpf(1.0f, kBExp - eB);
-  float* kt = smem;                                                   // FP32
-  uint32_t* kp = reinterpret_cast<uint32_t*>(smem + tableWords);      // FP16 (h, l)
-  int* stPos = reinterpret_cast<int*>(kp + tableWords);               // [wave][64]
+  const int tableWords = GT ? 0 : (OS * rowLen + 3) & ~3;
+  const float* kt = GT ? gkt : smem;                                          // FP32
+  const uint32_t* kp = GT ? gkp : reinterpret_cast<const uint32_t*>(smem + tableWords);
+  int* stPos = reinterpret_cast<int*>(smem + 2 * tableWords);          // [wave][64]
   float2* stVis = reinterpret_cast<float2*>(stPos + kMfmaWaves * 64);  // [wave][64][NIMG]
-  for (int i = threadIdx.x; i < OS * rowLen; i += blockDim.x) {
-    const int row = i / rowLen, c = i - row * rowLen - (T - 1);
-    float v = 0.0f;
-    if (c >= 0 && c < K) v = ktab[row * K + c];
-    const float vs = v * bScale;
-    const _Float16 h = (_Float16)vs;
-    const _Float16 l = (_Float16)(vs - (float)h);
-    kt[i] = v;
-    kp[i] = uint32_t(__builtin_bit_cast(uint16_t, h)) |
-            (uint32_t(__builtin_bit_cast(uint16_t, l)) << 16);
+  if (!GT) {
+    const float bScale = ldexpf(1.0f, kBExp - eB);
+    uint32_t* ktw = reinterpret_cast<uint32_t*>(smem + tableWords);
+    for (int i = threadIdx.x; i < OS * rowLen; i += blockDim.x)
+      f16TableEntry(i, ktab, K, rowLen, bScale, smem[i], ktw[i]);
+    __syncthreads();
   }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;  // A row / B column, k group
   int* myPos = stPos + wave * 64;
@@ -1633,8 +1660,10 @@ gridMfmaF16Kernel(GridArgs a, const uint32_t* __restrict__ binStart,
         for (int t = 0; t < 8; ++t) {
           // B[k][x = li] = xKernel, A[y = li][k] = v * yKernel
           // (yKernelValue * xKernel[i], gridding/wstackinggridder.cpp:574)
-          pr[t] = kp[(ps[t] & 0xFFFF) + li];
-          kyv[t] = kt[(ps[t] >> 16) + li];
+          // (unsigned: a y offset of 2^15 or more, OS above 885 at K = 7,
+          // sets the word's sign bit)
+          pr[t] = kp[(uint32_t(ps[t]) & 0xFFFFu) + li];
+          kyv[t] = kt[(uint32_t(ps[t]) >> 16) + li];
         }
         half8 bh, bl;
 #pragma unroll
@@ -2475,9 +2504,16 @@ bool mfmaKernelUsable(const hws_gridder* g) {
   return g->plan.tileSize == 16 && cellTablePadded(g->K, g->OS, 16) && !g->noMfma;
 }
 
+// The padded table of the FP16 matrix-core kernel too large for its LDS, but
+// indexable by its 16-bit record offsets: read from device memory.
+bool f16TableGlobal(const hws_gridder* g) {
+  return !cellTablePadded(g->K, g->OS, 16) && size_t(g->OS) * paddedRow(g->K, 16) < 65536;
+}
+
 // The mixed precision grids on the FP16 matrix cores (split operands).
 bool mfmaF16(const hws_gridder* g) {
-  return mfmaKernelUsable(g) && g->cfg.precision == HWS_PRECISION_MIXED && !g->noF16Mfma;
+  return g->plan.tileSize == 16 && !g->noMfma && g->cfg.precision == HWS_PRECISION_MIXED &&
+         !g->noF16Mfma && (cellTablePadded(g->K, g->OS, 16) || f16TableGlobal(g));
 }
 
 size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
@@ -2488,10 +2524,12 @@ size_t gridLdsBytes(const hws_gridder* g, int nimg, bool degrid) {
     return kt + size_t(nimg) * sg * sg * sizeof(float2) +
            size_t(kThreads / 64) * kDegridQueue * sizeof(uint2);
   }
-  if (mfmaKernelUsable(g)) {
+  if (mfmaKernelUsable(g) || mfmaF16(g)) {
     // the FP32 table, and for the FP16 kernel the (hi, lo) pairs beside it
-    const size_t table = size_t((g->OS * paddedRow(g->K, T) + 3) & ~3) * sizeof(float) *
-                         (mfmaF16(g) ? 2 : 1);
+    // (none where that kernel reads them from device memory)
+    size_t table = size_t((g->OS * paddedRow(g->K, T) + 3) & ~3) * sizeof(float) *
+                   (mfmaF16(g) ? 2 : 1);
+    if (mfmaF16(g) && f16TableGlobal(g)) table = 0;
     return table + kMfmaWaves * 64 * (sizeof(int) + size_t(nimg) * sizeof(float2));
   }
   if (cellKernelTile(T)) {
@@ -2535,6 +2573,18 @@ uint32_t workChunk(const hws_gridder* g, size_t nSamples) {
 }
 
 }  // namespace
+
+hws_status buildF16Table(hws_gridder* g) {
+  if (!f16TableGlobal(g) || g->cfg.precision != HWS_PRECISION_MIXED) return HWS_OK;
+  const size_t n = size_t(g->OS) * paddedRow(g->K, 16);
+  HWS_TRY(checkHip(g, hipMalloc(&g->dF16Table, 2 * n * sizeof(float)), "padded kernel table"));
+  hipLaunchKernelGGL(f16TableKernel, dim3(unsigned((n + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, g->stream, g->dKernel, g->K, g->OS, g->kernelExp,
+                     g->dF16Table, reinterpret_cast<uint32_t*>(g->dF16Table + n));
+  HWS_TRY(checkHip(g, hipGetLastError(), "padded kernel table launch"));
+  return checkHip(g, hipStreamSynchronize(g->stream), "padded kernel table");
+}
+
 
 hws_status launchFill(hws_gridder* g, void* ptr, uint32_t value, size_t n32) {
   if (n32 == 0) return HWS_OK;
@@ -2843,18 +2893,30 @@ hws_status gridBinned(hws_gridder* g, const float* dVis, int pass, bool accumula
         HWS_TRY(checkHip(g, hipGetLastError(), "layer clear launch"));
       }
       // partial tiles of a launch hold its own images only
-      if (mfmaKernelUsable(g)) {
+      if (mfmaKernelUsable(g) || mfmaF16(g)) {
         const unsigned mblocks = unsigned(std::max<uint64_t>(
             1, std::min<uint64_t>((kp.nWorkItems + kMfmaWaves - 1) / kMfmaWaves,
                                   uint64_t(persistentBlocks(g, 8)))));
         hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, g->stream, pl.queue, 0u, size_t(1));
-        const bool f16 = mfmaF16(g);
+        const bool f16 = mfmaF16(g), gt = f16 && f16TableGlobal(g);
+        if (gt && !g->dF16Table)
+          return setLastError(g, HWS_ERR_BAD_STATE, "padded kernel table not built");
+        const float* gkt = g->dF16Table;
+        const uint32_t* gkp =
+            gt ? reinterpret_cast<const uint32_t*>(g->dF16Table + size_t(g->OS) *
+                                                                      paddedRow(g->K, 16))
+               : nullptr;
 #define HWS_MFMA_GO(N, ACC, ROT)                                                               \
-  if (f16)                                                                                     \
-    hipLaunchKernelGGL((gridMfmaF16Kernel<N, ACC, ROT>), dim3(mblocks), dim3(kMfmaThreads),     \
-                       lds, g->stream, a, kp.binStart, kp.extraStart, kp.itemStart,             \
-                       kp.itemBin, kp.recSorted, vis, g->dKernel, layers, kp.partials,          \
-                       pl.queue, g->kernelExp);                                                 \
+  if (gt)                                                                                      \
+    hipLaunchKernelGGL((gridMfmaF16Kernel<N, ACC, ROT, true>), dim3(mblocks),                   \
+                       dim3(kMfmaThreads), lds, g->stream, a, kp.binStart, kp.extraStart,       \
+                       kp.itemStart, kp.itemBin, kp.recSorted, vis, g->dKernel, gkt, gkp,       \
+                       layers, kp.partials, pl.queue, g->kernelExp);                            \
+  else if (f16)                                                                                \
+    hipLaunchKernelGGL((gridMfmaF16Kernel<N, ACC, ROT, false>), dim3(mblocks),                  \
+                       dim3(kMfmaThreads), lds, g->stream, a, kp.binStart, kp.extraStart,       \
+                       kp.itemStart, kp.itemBin, kp.recSorted, vis, g->dKernel, nullptr,        \
+                       nullptr, layers, kp.partials, pl.queue, g->kernelExp);                   \
   else                                                                                         \
     hipLaunchKernelGGL((gridMfmaKernel<N, ACC, ROT>), dim3(mblocks), dim3(kMfmaThreads), lds,   \
                        g->stream, a, kp.binStart, kp.extraStart, kp.itemStart, kp.itemBin,      \
