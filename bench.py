@@ -57,6 +57,9 @@ def parse():
                    help="run only the facet-batch leg and print its JSON object")
     p.add_argument("--facet-scale", type=float, default=0.25,
                    help="fraction of the 1800 timesteps in the facet leg")
+    p.add_argument("--padded-only", metavar="SIZE,LAYERS,OS",
+                   help="run only this workload gridded at a padded size (e.g. 6144,128,1023, the "
+                        "setting that meets 1e-5) and print its JSON object")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -504,6 +507,16 @@ def main():
         return
     t0 = time.perf_counter()
     w = Workload(args.scale, dev)
+    if args.padded_only:
+        size, n_w, os_ = (int(v) for v in args.padded_only.split(","))
+        ms, npass, ph, _, _, _, _ = run_form(w, args.precision, args.steps, args.warmup, args.tile,
+                                             dev, args.chunk, cycle=False, n_w=n_w,
+                                             oversampling=os_, size=size)
+        print(json.dumps({"padded": {"gridded_size": size, "w_layers": n_w, "oversampling": os_,
+                                     "value": w.nvis / ms, "unit": "visibilities/s",
+                                     "ms_per_step": ms * 1e3, "passes": npass,
+                                     "phases_ms": {k: round(v, 3) for k, v in ph.items()}}}))
+        return
     log(f"workload ready in {time.perf_counter() - t0:.1f} s: {w.nrow} rows x {w.nchan} ch x 4 "
         f"= {w.nvis} vis, w in [{w.min_w:.1f}, {w.max_w:.1f}]")
 

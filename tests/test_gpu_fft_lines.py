@@ -1,14 +1,15 @@
 """The runtime-radix transforms of the blocked layer layout take four rows
-(columns) per workgroup where [4][n + 1] complex values fit the LDS, else two
-(a four-row block then spans two workgroups). Two lines are forced with
-HWS_FFT_TWO_LINES=1 when a context is created.
-- at small sizes, two lines against four: the transforms do the same
+(columns) per workgroup where they fit the LDS, else two or one (a four-row
+block then spans two or four workgroups). The count is forced with
+HWS_FFT_LINES=1, 2 or 4 when a context is created.
+- at small sizes, one and two lines against four: the transforms do the same
   operations on every line, so predicted visibilities are equal bit for bit;
   the images agree within 1e-6 of the peak (each context plans its samples
   itself, and the binning scatter may order a bin's records differently,
   which changes the gridded FP32 sums at rounding level);
-- sizes whose rows (or columns) take two lines by themselves (4800: four rows
-  need 153.6 KB of LDS) against the oracle, to the parity tolerances."""
+- sizes whose rows (or columns) take two lines or one by themselves (4800:
+  four rows would need 163 KB of LDS; 12288: two would need 209 KB) against
+  the oracle, to the parity tolerances."""
 import os
 
 import numpy as np
@@ -37,20 +38,20 @@ def _case(seed, width, nrow=300, nchan=3, dl=0.004, frac=0.42):
     return uvw, wl, vis, dl
 
 
-def _gpu(w, h, dl, uvw, wl, vis, nw, maxw, precision, model, two_lines):
+def _gpu(w, h, dl, uvw, wl, vis, nw, maxw, precision, model, lines):
     import torch
     assert torch.cuda.is_available(), "GPU tests need a HIP device"
     import wsclean_amd
-    old = os.environ.get("HWS_FFT_TWO_LINES")
-    os.environ["HWS_FFT_TWO_LINES"] = "1" if two_lines else "0"
+    old = os.environ.get("HWS_FFT_LINES")
+    os.environ["HWS_FFT_LINES"] = str(lines)
     try:
         g = wsclean_amd.WStackingGridder(w, h, dl, dl, 7, 63, precision=precision)
         p = wsclean_amd.WStackingGridder(w, h, dl, dl, 7, 63, precision=precision)
     finally:
         if old is None:
-            del os.environ["HWS_FFT_TWO_LINES"]
+            del os.environ["HWS_FFT_LINES"]
         else:
-            os.environ["HWS_FFT_TWO_LINES"] = old
+            os.environ["HWS_FFT_LINES"] = old
     g.prepare_w_layers(nw, 0.0, 0.0, maxw)
     g.start_inversion_pass(0)
     g.add_data(uvw, wl, vis)
@@ -67,13 +68,14 @@ def _gpu(w, h, dl, uvw, wl, vis, nw, maxw, precision, model, two_lines):
 
 @pytest.mark.parametrize("shape,precision", [((144, 112), "mixed"), ((96, 80), "reference"),
                                              ((480, 240), "mixed")])
-def test_two_lines_per_workgroup_equal_four(shape, precision):
+@pytest.mark.parametrize("lines", [1, 2])
+def test_fewer_lines_per_workgroup_equal_four(shape, precision, lines):
     w, h = shape
     uvw, wl, vis, dl = _case(31, min(w, h))
     maxw = float(np.abs(uvw[:, 2]).max() / wl.min())
     model = np.random.default_rng(3).standard_normal((h, w)).astype(np.float32)
-    re4, v4 = _gpu(w, h, dl, uvw, wl, vis, 6, maxw, precision, model, False)
-    re2, v2 = _gpu(w, h, dl, uvw, wl, vis, 6, maxw, precision, model, True)
+    re4, v4 = _gpu(w, h, dl, uvw, wl, vis, 6, maxw, precision, model, 4)
+    re2, v2 = _gpu(w, h, dl, uvw, wl, vis, 6, maxw, precision, model, lines)
     assert np.abs(re4).max() > 0
     assert np.abs(re2 - re4).max() <= 1e-6 * np.abs(re4).max()
     assert np.array_equal(np.isnan(v2), np.isnan(v4))
@@ -81,8 +83,10 @@ def test_two_lines_per_workgroup_equal_four(shape, precision):
     assert ok.sum() > 0 and np.array_equal(v2[ok], v4[ok])
 
 
-@pytest.mark.parametrize("shape,precision", [((4800, 160), "mixed"), ((160, 4800), "reference")])
-def test_two_line_sizes_match_oracle(oracle, shape, precision):
+@pytest.mark.parametrize("shape,precision", [((4800, 160), "mixed"), ((160, 4800), "reference"),
+                                             ((12288, 64), "mixed"), ((64, 12288), "mixed")])
+def test_long_line_sizes_match_oracle(oracle, shape, precision):
+    # 4800: two lines per workgroup; 12288 (= 3 x 4096): one
     w, h = shape
     # (an image 0.4 rad across its long side, inside the horizon)
     uvw, wl, vis, dl = _case(41, min(w, h), nrow=200, dl=0.4 / max(w, h))
@@ -102,7 +106,7 @@ def test_two_line_sizes_match_oracle(oracle, shape, precision):
     op.initialize_prediction(model, None)
     op.start_prediction_pass(0)
     v_o = op.sample(uvw_l)
-    re, v = _gpu(w, h, dl, uvw, wl, vis, nw, maxw, precision, model, False)
+    re, v = _gpu(w, h, dl, uvw, wl, vis, nw, maxw, precision, model, 0)
     e_img = rel(re, img_o)
     ok = ~np.isnan(v_o)
     e_v = rel(v[ok], v_o[ok])

@@ -411,8 +411,9 @@ hws_status hws_create(const hws_config* c, hws_gridder** out) {
     g->noFusedProject = f && f[0] == '1';
     const char* h = std::getenv("HWS_NO_F16_MFMA");
     g->noF16Mfma = h && h[0] == '1';
-    const char* tl = std::getenv("HWS_FFT_TWO_LINES");
-    g->forceTwoLines = tl && tl[0] == '1';
+    const char* tl = std::getenv("HWS_FFT_LINES");
+    if (tl && (std::atoi(tl) == 1 || std::atoi(tl) == 2 || std::atoi(tl) == 4))
+      g->fftLines = std::atoi(tl);
     const char* d = std::getenv("HWS_DEGRID_BLOCKS_PER_CU");
     if (d && std::atoi(d) > 0 && std::atoi(d) <= 16) g->degridBlocksPerCu = std::atoi(d);
   }

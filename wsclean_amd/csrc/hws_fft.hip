@@ -1048,8 +1048,8 @@ __device__ __forceinline__ void blockedRtStage(float2* lds, int n, int ns,
                                                const float2* __restrict__ tw, int tid, int nthr,
                                                const Load& load, const Store& store) {
   constexpr int kBatches = kMaxValues / R;
-  constexpr int LS = L == 4 ? 2 : 1;
-  static_assert(L == 2 || L == 4, "lines");
+  constexpr int LS = L == 4 ? 2 : L == 2 ? 1 : 0;
+  static_assert(L == 1 || L == 2 || L == 4, "lines");
   // (an opaque tid keeps the compiler from hoisting the batch indices of
   // every radix out of the stage loop, which keeps them all live)
   asm volatile("" : "+v"(tid));
@@ -1152,14 +1152,17 @@ __device__ __forceinline__ void blockedRtFft(float2* lds, int n, unsigned long l
   blockedRtStageR<SIGN, L, false, true>(R, lds, n, ns, tw, tid, nthr, fromLds, store);
 }
 
-// Two lines per workgroup: the two workgroups of a four-line block read and
-// write the two halves of the same 32-byte (rows) or 128-byte (columns)
-// segments. The hardware sends workgroup b to XCD b % 8, so within every 16
-// workgroups b and b + 8 are given the two halves of one block: they run on
-// the same XCD at about the same time and share its L2.
-__device__ __forceinline__ int pairedBlock(int b, int nBlocks) {
-  if ((nBlocks & 15) != 0) return b;
-  return (b & ~15) + ((b & 7) << 1) + ((b >> 3) & 1);
+// Fewer than four lines per workgroup: the workgroups of a four-line block
+// read and write parts of the same 32-byte (rows) or 128-byte (columns)
+// segments. The hardware sends workgroup b to XCD b % 8, so within every
+// 8 * P workgroups (P = 4 / L parts per block) b, b + 8, ... are given the
+// parts of one block: they run on the same XCD at about the same time and
+// share its L2.
+template <int L>
+__device__ __forceinline__ int sharedSegmentBlock(int b, int nBlocks) {
+  constexpr int P = 4 / L, G = 8 * P;
+  if (P == 1 || nBlocks % G != 0) return b;
+  return (b / G) * G + (b & 7) * P + ((b >> 3) & (P - 1));
 }
 
 template <int SIGN, int MODE, bool REF, int L>
@@ -1169,8 +1172,8 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
                             GenericSparseArgs sp, ModelSource ms) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int rblk = L == 2 ? pairedBlock(int(blockIdx.x), int(gridDim.x)) : int(blockIdx.x);
-  const int r0 = rblk * L;                // rows of this workgroup: 4, or 2 (half a row block)
+  const int rblk = sharedSegmentBlock<L>(int(blockIdx.x), int(gridDim.x));
+  const int r0 = rblk * L;                // rows of this workgroup: 4, 2 or 1 of a row block
   const int f0 = r0 & 3;                  // their position in the four-row block
   const int layer = sp.layer0 + int(blockIdx.y);
   const int ty = r0 >> sp.tshift;
@@ -1203,10 +1206,10 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
   // lie near the uv origin, at both ends of the row)
   if (MODE != 2 && (nColBlocks & 7) == 0)
     blk = (blk & 7) * (nColBlocks >> 3) + (blk >> 3);
-  else if (MODE == 2 && L == 2)
-    blk = pairedBlock(blk, nColBlocks);
+  else if (MODE == 2)
+    blk = sharedSegmentBlock<L>(blk, nColBlocks);
   const int layer = sp.layer0 + int(blockIdx.y);
-  const int c0 = blk * L;  // columns of this workgroup: 4, or 2
+  const int c0 = blk * L;  // columns of this workgroup: 4, 2 or 1
   const int tx = c0 >> sp.tshift;
   // MODE 1: bit (layer, tile row); MODE 2: bit (layer, tile row, this tile column)
   auto live = [&](int y) -> bool {
@@ -1239,7 +1242,7 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
 template <int R, int L, class Visit>
 __device__ __forceinline__ void firstStageCells(int n, int tid, int nthr, const Visit& visit) {
   constexpr int kBatches = kMaxValues / R;
-  constexpr int LS = L == 4 ? 2 : 1;
+  constexpr int LS = L == 4 ? 2 : L == 2 ? 1 : 0;
   asm volatile("" : "+v"(tid));
   const int B = n / R;
   const int total = B * L;
@@ -1270,7 +1273,7 @@ fftRowsModelLayersGenericKernel(float2* __restrict__ data, int W, int H, int nba
                                 int tshift) {
   extern __shared__ float2 lds[];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int rblk = L == 2 ? pairedBlock(int(blockIdx.x), int(gridDim.x)) : int(blockIdx.x);
+  const int rblk = sharedSegmentBlock<L>(int(blockIdx.x), int(gridDim.x));
   const int r0 = rblk * L;
   const size_t HW = size_t(W) * H;
   float2* rowBase = data + size_t(r0 >> 2) * 4 * W + (r0 & 3);
@@ -1327,8 +1330,11 @@ void launchRowsGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_
   if (lines == 4)
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 4>), grid, dim3(threads),
                        lds, st, data, W, H, rad, tw, sp, ms);
-  else
+  else if (lines == 2)
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 2>), grid, dim3(threads),
+                       lds, st, data, W, H, rad, tw, sp, ms);
+  else
+    hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 1>), grid, dim3(threads),
                        lds, st, data, W, H, rad, tw, sp, ms);
 }
 
@@ -1343,9 +1349,12 @@ void launchModelLayersGeneric(int threads, int lines, dim3 grid, size_t lds, hip
   if (lines == 4) {
     if (cplx) HWS_MODEL_GENERIC(true, 4);
     else HWS_MODEL_GENERIC(false, 4);
-  } else {
+  } else if (lines == 2) {
     if (cplx) HWS_MODEL_GENERIC(true, 2);
     else HWS_MODEL_GENERIC(false, 2);
+  } else {
+    if (cplx) HWS_MODEL_GENERIC(true, 1);
+    else HWS_MODEL_GENERIC(false, 1);
   }
 #undef HWS_MODEL_GENERIC
 }
@@ -1357,8 +1366,11 @@ void launchColsGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_
   if (lines == 4)
     hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 4>), grid, dim3(threads), lds,
                        st, data, W, H, nColBlocks, rad, tw, sp);
-  else
+  else if (lines == 2)
     hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 2>), grid, dim3(threads), lds,
+                       st, data, W, H, nColBlocks, rad, tw, sp);
+  else
+    hipLaunchKernelGGL((fftColsBlockedGenericKernel<SIGN, MODE, 1>), grid, dim3(threads), lds,
                        st, data, W, H, nColBlocks, rad, tw, sp);
 }
 
@@ -1532,32 +1544,39 @@ bool fastBlockedSize(int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0
 // less room for the kernels' static arrays).
 constexpr size_t kGenericLdsBytes = size_t(152) * 1024;
 
-// The blocked layout is used whenever both sizes are multiples of 4 and two
-// lines of each fit the LDS of one runtime-radix workgroup (n <= 9152); the
+// The blocked layout is used whenever both sizes are multiples of 4 and one
+// line of each fits the LDS of one runtime-radix workgroup (n <= 18308); the
 // transform plans then also have to fit the workgroup's threads
-// (blockedGenericFits).
+// (blockedGenericFits: 16 values per thread, n up to about 15000).
 bool blockedLayoutSupported(int W, int H) {
   if (fastBlockedSize(W) && fastBlockedSize(H)) return true;
-  auto fits = [](int n) { return blockedGenericLds(2, n) <= kGenericLdsBytes; };
+  auto fits = [](int n) { return blockedGenericLds(1, n) <= kGenericLdsBytes; };
   return W % 4 == 0 && H % 4 == 0 && W >= 16 && H >= 16 && fits(W) && fits(H);
 }
 
 namespace {
 // log2 of the lines (rows or columns) a runtime-radix workgroup of the
-// blocked layout transforms: the four of a row block where they fit the LDS
-// and the plan the threads, else two; -1 if two do not fit.
-int genericLinesShift(const FftPlan& plan, bool forceTwo) {
+// blocked layout transforms: the four of a row block where two such
+// workgroups fit the LDS of a CU and the plan the threads, else two, else
+// one; -1 if one does not fit.
+int genericLinesShift(const FftPlan& plan, int force) {
   auto fits = [&](int lines) {
     return blockedGenericLds(lines, plan.n) <= kGenericLdsBytes &&
            threadsFor(plan, lines) != 0;
   };
-  if (!forceTwo && fits(4)) return 2;
-  return fits(2) ? 1 : -1;
+  if (force && fits(force)) return force == 4 ? 2 : force == 2 ? 1 : 0;
+  // (four lines only while two workgroups fit a CU: at 3072, four lines in
+  // one workgroup per CU took 26.6 / 19.5 ms per step for the backward /
+  // forward transforms, two lines in three 25.1 / 18.5; one line, whose
+  // workgroups read a quarter of each segment, 27.9 / 28.8)
+  if (fits(4) && blockedGenericLds(4, plan.n) <= kGenericLdsBytes / 2) return 2;
+  if (fits(2)) return 1;
+  return fits(1) ? 0 : -1;
 }
 }  // namespace
 
 bool blockedGenericFits(const hws_gridder* g) {
-  return genericLinesShift(g->rowPlan, false) >= 0 && genericLinesShift(g->colPlan, false) >= 0;
+  return genericLinesShift(g->rowPlan, 0) >= 0 && genericLinesShift(g->colPlan, 0) >= 0;
 }
 
 namespace {
@@ -1630,9 +1649,9 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     // kernels)
     const float2* twRow = sign > 0 ? g->rowPlan.twBackward : g->rowPlan.twForward;
     const float2* twCol = sign > 0 ? g->colPlan.twBackward : g->colPlan.twForward;
-    // four rows (columns) per workgroup where they fit the LDS, else two
-    const int rbShift = genericLinesShift(g->rowPlan, g->forceTwoLines);
-    const int cbShift = genericLinesShift(g->colPlan, g->forceTwoLines);
+    // four rows (columns) per workgroup where they fit the LDS, else two, else one
+    const int rbShift = genericLinesShift(g->rowPlan, g->fftLines);
+    const int cbShift = genericLinesShift(g->colPlan, g->fftLines);
     if (rbShift < 0 || cbShift < 0)
       return setLastError(g, HWS_ERR_UNSUPPORTED, "transform does not fit one workgroup");
     const int tr = threadsFor(g->rowPlan, 1 << rbShift), tc = threadsFor(g->colPlan, 1 << cbShift);

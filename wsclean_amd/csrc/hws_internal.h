@@ -387,9 +387,14 @@ struct hws_gridder {
   // grid with the VALU cell kernel at T = 16 instead of the matrix-core one
   // (HWS_NO_MFMA=1 in the environment; for comparisons)
   bool noMfma = false;
+  // the mixed precision grids with the FP32 matrix-core kernel
+  // (HWS_NO_F16_MFMA=1)
   bool noF16Mfma = false;
+  // degridding workgroups per CU (HWS_DEGRID_BLOCKS_PER_CU)
   int degridBlocksPerCu = 5;
-  bool forceTwoLines = false;  // runtime-radix transforms: two lines per workgroup  // the mixed precision grids with the FP32 matrix-core kernel
+  // lines (rows or columns) per workgroup of the blocked runtime-radix
+  // transforms (HWS_FFT_LINES = 1, 2 or 4; 0: chosen by size)
+  int fftLines = 0;
   // run the backward columns pass and the projection apart, and the forward
   // rows pass four layers per workgroup (the forms the fused kernels replace)
   bool noFusedProject = false;
