@@ -1,7 +1,8 @@
 """The fused layer-transform kernels against the kernels they replace (real
 and complex images, one pass and several, both batched precisions, every
-transform size the fused kernels cover); the replaced kernels are selected
-with HWS_NO_FUSED_PROJECT=1 when a context is created.
+power-of-two transform size the fused kernels cover and runtime-radix sizes
+up to 6144); the replaced kernels are selected with HWS_NO_FUSED_PROJECT=1
+when a context is created.
 - predict: the forward rows pass over every layer per workgroup from the
   model (fftRowsModelLayersKernel) against four layers per workgroup
   (fftRowsFromModelKernel). Degridding does not depend on the order of a
@@ -78,6 +79,9 @@ def _run(w, h, dl, uvw, wl, vis, maxw, precision, nw, max_mem, fused):
 @pytest.mark.parametrize("w,h,precision,nw,passes", [
     (256, 256, "mixed", 8, 1), (512, 256, "reference", 6, 1), (1024, 1024, "mixed", 12, 3),
     (2048, 2048, "mixed", 8, 1), (4096, 4096, "mixed", 6, 2), (1024, 512, "reference", 9, 2),
+    # runtime-radix sizes (fftColsProjectGenericKernel, fftRowsModelLayersGenericKernel)
+    (480, 240, "mixed", 8, 1), (960, 720, "reference", 6, 2), (1536, 1536, "mixed", 12, 3),
+    (512, 6144, "mixed", 4, 1),
 ])
 def test_fused_projection_equals_separate_kernels(w, h, precision, nw, passes):
     uvw, wl, vis, dl, maxw = _case(min(w, h), 400, seed=w + h + nw)

@@ -1041,8 +1041,8 @@ __host__ __device__ constexpr size_t blockedGenericLds(int L, int n) {
 // y[(b/ns)*ns*R + jm + r*ns]. The first stage (ns == 1) reads its inputs with
 // load(slot, pos, f) instead of from LDS (slot = q * R + r, this thread's
 // q-th butterfly's r-th input, below kMaxValues) and needs no barrier before
-// its writes; the last one writes its outputs with store(pos, f, v) instead
-// of to LDS.
+// its writes; the last one writes its outputs with store(slot, pos, f, v)
+// instead of to LDS.
 template <int SIGN, int R, int L, bool FIRST, bool LAST, class Load, class Store>
 __device__ __forceinline__ void blockedRtStage(float2* lds, int n, int ns,
                                                const float2* __restrict__ tw, int tid, int nthr,
@@ -1091,7 +1091,7 @@ __device__ __forceinline__ void blockedRtStage(float2* lds, int n, int ns,
       for (int r = 0; r < R; ++r) {
         const int pos = idxD + r * ns;
         if (LAST)
-          store(pos, f, v[q][r]);
+          store(q * R + r, pos, f, v[q][r]);
         else
           lds[ipad(pos * L + f)] = v[q][r];
       }
@@ -1134,7 +1134,7 @@ __device__ __forceinline__ void blockedRtFft(float2* lds, int n, unsigned long l
   };
   int R = radixOf(radices);
   radices >>= 4;
-  auto toLds = [](int, int, float2) {};
+  auto toLds = [](int, int, int, float2) {};
   auto fromLds = [](int, int, int) { return make_float2(0.0f, 0.0f); };
   if (radices == 0ull) {  // a single stage: memory to memory
     blockedRtStageR<SIGN, L, true, true>(R, lds, n, 1, tw, tid, nthr, load, store);
@@ -1190,7 +1190,7 @@ fftRowsBlockedGenericKernel(float2* __restrict__ data, int W, int H,
     if (MODE == 1 && !bitAt(sp.bits, rowBase + (x >> sp.tshift))) return make_float2(0.0f, 0.0f);
     return base[size_t(x) * 4 + f];
   };
-  auto store = [&](int x, int f, float2 v) { base[size_t(x) * 4 + f] = v; };
+  auto store = [&](int, int x, int f, float2 v) { base[size_t(x) * 4 + f] = v; };
   blockedRtFft<SIGN, L>(lds, W, radices, tw, tid, nthr, load, store);
 }
 
@@ -1229,7 +1229,7 @@ fftColsBlockedGenericKernel(float2* __restrict__ data, int W, int H, int nColBlo
     if (MODE == 1 && !live(y)) return make_float2(0.0f, 0.0f);
     return base[addr(y, f)];
   };
-  auto store = [&](int y, int f, float2 v) {
+  auto store = [&](int, int y, int f, float2 v) {
     if (MODE == 2 && !live(y)) return;
     base[addr(y, f)] = v;
   };
@@ -1314,7 +1314,7 @@ fftRowsModelLayersGenericKernel(float2* __restrict__ data, int W, int H, int nba
     auto load = [&](int slot, int, int) -> float2 {
       return modelTerm<REF>(re[slot], CPLX ? imv[slot] : 0.0f, CPLX, w, np[slot]);
     };
-    auto store = [&](int x, int f, float2 v) {
+    auto store = [&](int, int x, int f, float2 v) {
       const size_t bit = cbit + size_t(x >> tshift);
       if ((colNeed[bit >> 5] >> (bit & 31)) & 1u) streamStore(&base[size_t(x) * 4 + f], v);
     };
@@ -1336,6 +1336,124 @@ void launchRowsGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_
   else
     hipLaunchKernelGGL((fftRowsBlockedGenericKernel<SIGN, MODE, REF, 1>), grid, dim3(threads),
                        lds, st, data, W, H, rad, tw, sp, ms);
+}
+
+// Backward columns pass fused with the projection at the runtime-radix
+// sizes (fftColsProjectKernel at the power-of-two sizes): a workgroup owns
+// one column of one image and transforms it layer after layer, in the pass's
+// layer order; each transformed cell is rotated by its layer's w term and
+// added (Kahan) to its pixel's sums, which stay in registers with the
+// pixel's n' (the cells of the last stage's slots). Cells of tile rows
+// without samples read as zero. Every term is computed and added as
+// projectKernel computes and adds it. grid (W / 1, images of the kind);
+// dynamic LDS: the transform buffer, then the row bits of the pass's layers.
+template <bool REF, bool CPLX>
+__global__ void __launch_bounds__(512)
+fftColsProjectGenericKernel(ColsProjectArgs a, const float2* __restrict__ layers,
+                            unsigned long long radices, const float2* __restrict__ tw,
+                            float* __restrict__ sum, float* __restrict__ comp,
+                            float* __restrict__ isum, float* __restrict__ icomp) {
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int W = a.W, H = a.H;
+  const size_t HW = size_t(W) * H;
+  const int img = a.img[blockIdx.y];
+  const bool fresh = a.fresh != 0;
+  // the four columns of a 128-byte segment run on one XCD, one after the other
+  const int nb = int(gridDim.x);
+  int x0 = int(blockIdx.x);
+  if ((nb & 7) == 0) x0 = (x0 & 7) * (nb >> 3) + (x0 >> 3);
+  uint32_t* bits =
+      reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(lds) + blockedGenericLds(1, H));
+  const int nl = a.nLayersInPass;
+  {
+    const size_t nbits = size_t(nl) * a.tilesY;
+    const size_t bit0 = size_t(a.passStart) * a.tilesY;
+    const size_t w0 = bit0 >> 5;
+    const int sh = int(bit0 & 31);
+    for (size_t wd = size_t(tid); wd < (nbits + 31) / 32; wd += size_t(nthr)) {
+      uint32_t v = a.rowLive[w0 + wd] >> sh;
+      if (sh != 0 && w0 + wd + 1 < a.rowLiveWords) v |= a.rowLive[w0 + wd + 1] << (32 - sh);
+      bits[wd] = v;
+    }
+  }
+  // this thread's output pixels (the last stage's slots): n' and the sums
+  float s[kMaxValues], c[kMaxValues], is[CPLX ? kMaxValues : 1], ic[CPLX ? kMaxValues : 1];
+  double np[kMaxValues];
+  auto pixel = [&](int y) {
+    int ySrc = (H - y) + H / 2;
+    if (ySrc >= H) ySrc -= H;
+    int xSrc = x0 + W / 2;
+    if (xSrc >= W) xSrc -= W;
+    return size_t(img) * HW + size_t(ySrc) * W + xSrc;
+  };
+  auto init = [&](int slot, int y, int) {
+    int ySrc = (H - y) + H / 2;
+    if (ySrc >= H) ySrc -= H;
+    int xSrc = x0 + W / 2;
+    if (xSrc >= W) xSrc -= W;
+    const double m = pixelM(ySrc, H, a.dm, a.mShift);
+    const double l = pixelL(xSrc, W, a.dl, a.lShift);
+    double v = nPrime(l, m);
+    if (REF) v = double(float(v));
+    np[slot] = v;
+    const size_t o = pixel(y);
+    s[slot] = fresh ? 0.0f : sum[o];
+    c[slot] = fresh ? 0.0f : comp[o];
+    if (CPLX) {
+      is[slot] = fresh ? 0.0f : isum[o];
+      ic[slot] = fresh ? 0.0f : icomp[o];
+    }
+  };
+  auto finish = [&](int slot, int y, int) {
+    const size_t o = pixel(y);
+    sum[o] = s[slot];
+    comp[o] = c[slot];
+    if (CPLX) {
+      isum[o] = is[slot];
+      icomp[o] = ic[slot];
+    } else if (fresh && isum != nullptr) {
+      // a fresh pass also zeroes the imaginary accumulators of real images
+      isum[o] = 0.0f;
+      icomp[o] = 0.0f;
+    }
+  };
+  // the last stage's radix: the highest nonzero 4-bit code
+  int lastCode = 0;
+  for (unsigned long long r = radices; r != 0ull; r >>= 4) lastCode = int(r & 15ull);
+  auto overLastSlots = [&](const auto& visit) {
+    switch (lastCode) {
+      case 1: firstStageCells<16, 1>(H, tid, nthr, visit); break;
+      case 8: firstStageCells<8, 1>(H, tid, nthr, visit); break;
+      case 4: firstStageCells<4, 1>(H, tid, nthr, visit); break;
+      case 2: firstStageCells<2, 1>(H, tid, nthr, visit); break;
+      case 3: firstStageCells<3, 1>(H, tid, nthr, visit); break;
+      case 5: firstStageCells<5, 1>(H, tid, nthr, visit); break;
+      case 7: firstStageCells<7, 1>(H, tid, nthr, visit); break;
+      default: break;
+    }
+  };
+  overLastSlots(init);
+  __syncthreads();  // the row bits
+  const float2* col = layers + size_t(img) * a.nLayersPass * HW + size_t(x0) * 4;
+#pragma unroll 1
+  for (int li = 0; li < nl; ++li) {
+    const float2* lay = col + size_t(li) * HW;
+    const double w = layerToW(a.map, a.passStart + li);
+    auto load = [&](int, int y, int) -> float2 {
+      const int bit = li * a.tilesY + (y >> a.tshift);
+      if (!((bits[bit >> 5] >> (bit & 31)) & 1u)) return make_float2(0.0f, 0.0f);
+      return lay[size_t(y >> 2) * W * 4 + (y & 3)];
+    };
+    auto project = [&](int slot, int, int, float2 v) {
+      const float2 t = projectTerm<REF>(v, w, np[slot]);
+      kahanAdd(s[slot], c[slot], t.x);
+      if (CPLX) kahanAdd(is[slot], ic[slot], t.y);
+    };
+    blockedRtFft<+1, 1>(lds, H, radices, tw, tid, nthr, load, project);
+    __syncthreads();  // the last stage's LDS reads precede the next layer's writes
+  }
+  overLastSlots(finish);
 }
 
 void launchModelLayersGeneric(int threads, int lines, dim3 grid, size_t lds, hipStream_t st,
@@ -1588,13 +1706,28 @@ size_t colsProjectLds(int H, int nLayersInPass, int tilesY) {
 }
 }  // namespace
 
+// The runtime-radix fused kernel: one column per workgroup of at most 512
+// threads; its LDS is the transform buffer and the row bits.
+size_t colsProjectGenericLds(int H, int nLayersInPass, int tilesY) {
+  const size_t words = (size_t(nLayersInPass) * tilesY + 31) / 32;
+  return blockedGenericLds(1, H) + words * sizeof(uint32_t);
+}
+
+bool colsProjectGeneric(const hws_gridder* g) {
+  return !(fastBlockedSize(g->W) && fastBlockedSize(g->H));
+}
+
 bool colsProjectSupported(const hws_gridder* g, int nLayersInPass) {
-  if (!g->blocked || !fastBlockedSize(g->W) || !fastBlockedSize(g->H) || g->noFusedProject)
-    return false;
+  if (!g->blocked || g->noFusedProject) return false;
   const int T = g->plan.tileSize;
   if (T < 4 || (T & (T - 1)) != 0) return false;
   int maxLds = 65536;
   (void)hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, g->cfg.device);
+  if (colsProjectGeneric(g)) {
+    const int t = threadsFor(g->colPlan, 1);
+    return t != 0 && t <= 512 &&
+           colsProjectGenericLds(g->H, nLayersInPass, g->H / T) <= size_t(maxLds);
+  }
   return colsProjectLds(g->H, nLayersInPass, g->H / T) <= size_t(maxLds);
 }
 
@@ -1603,6 +1736,22 @@ hws_status launchColsProject(hws_gridder* g, const ColsProjectArgs& a, bool cplx
                              float* icomp) {
   if (a.nImg == 0 || a.nLayersInPass == 0) return HWS_OK;
   const bool ref = g->cfg.precision == HWS_PRECISION_REFERENCE;
+  if (colsProjectGeneric(g)) {
+    const dim3 grid(unsigned(a.W), unsigned(a.nImg));
+    const int threads = threadsFor(g->colPlan, 1);
+    const size_t lds = colsProjectGenericLds(a.H, a.nLayersInPass, a.tilesY);
+    const unsigned long long rad = packRadices(g->colPlan);
+    const float2* tw = g->colPlan.twBackward;
+#define HWS_COLS_PROJECT_GENERIC(R, C)                                                          \
+  hipLaunchKernelGGL((fftColsProjectGenericKernel<R, C>), grid, dim3(threads), lds, g->stream,   \
+                     a, layers, rad, tw, sum, comp, isum, icomp)
+    if (ref && cplx) HWS_COLS_PROJECT_GENERIC(true, true);
+    else if (ref) HWS_COLS_PROJECT_GENERIC(true, false);
+    else if (cplx) HWS_COLS_PROJECT_GENERIC(false, true);
+    else HWS_COLS_PROJECT_GENERIC(false, false);
+#undef HWS_COLS_PROJECT_GENERIC
+    return checkHip(g, hipGetLastError(), "fused columns and projection launch");
+  }
   const dim3 grid(unsigned(a.W / kProjectColumns), unsigned(a.nImg));
   const size_t lds = colsProjectLds(a.H, a.nLayersInPass, a.tilesY);
   const float2* tw = g->colPlan.twBackward;
@@ -1679,7 +1828,9 @@ hws_status fft2d(hws_gridder* g, float2* data, int nbatch, int sign, const FftSp
     if (sign > 0) {
       if (sparse) {
         launchRowsGeneric<+1, 1, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
-        launchColsGeneric<+1, 1>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
+        // (rowsOnly: the columns pass runs fused with the projection)
+        if (!sparse->rowsOnly)
+          launchColsGeneric<+1, 1>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
       } else {
         launchRowsGeneric<+1, 0, false>(tr, rl, gr, ldsR, st, data, W, H, radR, twRow, rs, msrc);
         launchColsGeneric<+1, 0>(tc, cl, gc, ldsC, st, data, W, H, nColBlocks, radC, twCol, cs);
